@@ -1,0 +1,179 @@
+/* zgl.h -- C ABI of the MI355X rendering library (libzgl.so).
+ *
+ * Drop-in replacement for the reference's compute-shader dispatch
+ * (the src/shader GLSL kernels driven by src/integrator/{NaivePath,LightPath,TriplePath}.cpp).
+ * Plain pointers and sizes only; the caller owns every host pointer and the
+ * library copies what it needs before returning.  No entry point aborts: each
+ * returns a zgl_status and, for context calls, leaves a message readable with
+ * zgl_last_error().  A context is not thread-safe (like the reference's single GL
+ * context); distinct contexts may be used from distinct threads.
+ */
+#ifndef ZGL_H
+#define ZGL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t zgl_status;
+#define ZGL_OK 0
+#define ZGL_E_INVALID 1  /* bad argument or inconsistent buffers */
+#define ZGL_E_HIP 2      /* HIP runtime error */
+#define ZGL_E_OOM 3      /* device or host allocation failed */
+#define ZGL_E_STATE 4    /* call order (e.g. render before upload) */
+#define ZGL_E_NODEVICE 5 /* no HIP device available */
+
+#define ZGL_INTEGRATOR_NAIVE 0  /* NaivePathIntegrator  (src/integrator/NaivePath.cpp) */
+#define ZGL_INTEGRATOR_LIGHT 1  /* LightPathIntegrator  (src/integrator/LightPath.cpp) */
+#define ZGL_INTEGRATOR_TRIPLE 2 /* TriplePathIntegrator (src/integrator/TriplePath.cpp) */
+
+/* ------------------------------------------------------------------ */
+/* Host producers (no GPU needed)                                       */
+/* ------------------------------------------------------------------ */
+
+/* BVH::build (src/accelerator/BVH.cpp:116-144): 16-bucket binned SAH, one triangle
+ * per leaf, threaded six ways (buildHitTable, :298-345).
+ * bounds_out: (2n-1)*6 floats; hit_table_out: 6*(2n-1)*3 ints. */
+zgl_status zgl_build_mtbvh(const float* vertices, int64_t n_vertices, const uint32_t* indices,
+                           int64_t n_triangles, float* bounds_out, int32_t* hit_table_out);
+
+/* AliasTable::build<int32_t> (src/math/AliasTable.h:11-57). */
+zgl_status zgl_alias_build(const float* pdf, int32_t n, int32_t* alias_out, float* prob_out);
+
+/* EnvironmentMap::EnvironmentMap + setupAliasTable (src/core/EnvironmentMap.cpp:8-114):
+ * alias/prob are (width+1)*height; *sum_out is the float sum whose int truncation is
+ * the reference's sumPdf(). */
+zgl_status zgl_env_alias_build(const float* rgb, int32_t width, int32_t height, int32_t* alias_out,
+                               float* prob_out, float* sum_out);
+
+/* Sampler::sobolSample (src/core/Sampler.cpp:19-28). */
+uint32_t zgl_sobol_sample(uint32_t index, int32_t dim, uint32_t scramble);
+/* SobolMatrices (src/math/SobolMatrices256x32.h): 256*32 generator columns. */
+void zgl_sobol_matrices(uint32_t* out);
+
+/* FP32 -> FP16 (round to nearest even) as done by the RGB16F upload of the env map
+ * (src/core/EnvironmentMap.cpp:13). */
+void zgl_float_to_half(const float* in, uint16_t* out, int64_t n);
+
+/* ------------------------------------------------------------------ */
+/* Device context                                                       */
+/* ------------------------------------------------------------------ */
+typedef struct zgl_context zgl_context;
+
+int32_t zgl_device_count(void);
+zgl_status zgl_create(int32_t device, zgl_context** out);
+void zgl_destroy(zgl_context* ctx);
+const char* zgl_last_error(const zgl_context* ctx);
+/* Use a caller-owned hipStream_t (NULL = the context's own stream). */
+zgl_status zgl_set_stream(zgl_context* ctx, void* hip_stream);
+void* zgl_get_stream(zgl_context* ctx);
+zgl_status zgl_synchronize(zgl_context* ctx);
+
+/* Scene buffers in the reference's layout: SceneGLContext (src/core/Scene.h:12-27)
+ * plus the scalars NaivePathIntegrator::updateUniforms binds
+ * (src/integrator/NaivePath.cpp:39-47). */
+typedef struct zgl_scene_buffers {
+    const float* vertices;        /* RGB32F, n_vertices */
+    const float* normals;         /* RGB32F, n_vertices */
+    int64_t n_vertices;
+    const float* texcoords;       /* RG32F, object vertices only */
+    int64_t n_texcoords;
+    const int32_t* indices;       /* 3 per triangle: objects first, then lights */
+    int64_t n_triangles;
+    const float* bounds;          /* bvh_size nodes x {pMin.xyz, pMax.xyz} */
+    const int32_t* hit_table;     /* 6 x bvh_size x {node, prim, miss} */
+    int32_t bvh_size;             /* boxCount = 2n - 1 */
+    const int32_t* mat_tex_indices; /* one per object triangle */
+    int32_t obj_prim_count;       /* objPrimCount */
+    const float* materials;       /* 16 floats per material; [13] holds the int type */
+    int32_t n_materials;
+    const float* light_power;     /* RGB32F per light triangle */
+    const int32_t* light_alias;
+    const float* light_prob;
+    int32_t n_light_triangles;
+    float light_sum;              /* lightSumPdf */
+    const uint16_t* env_rgb16f;   /* RGB16F, env_width x env_height, row 0 = v 0 */
+    int32_t env_width;
+    int32_t env_height;
+    const int32_t* env_alias;     /* (env_width+1) x env_height */
+    const float* env_prob;        /* (env_width+1) x env_height */
+    float env_sum;                /* (float)(int)sum, as sumPdf() returns int */
+    float env_rotation;           /* uEnvRotation */
+    int32_t sampler;              /* 0 independent, 1 Sobol (scene->sampler) */
+} zgl_scene_buffers;
+
+zgl_status zgl_upload_scene(zgl_context* ctx, const zgl_scene_buffers* scene);
+
+/* Camera uniforms (src/integrator/NaivePath.cpp:49-60). mat_inv is column-major. */
+typedef struct zgl_camera {
+    float front[3], right[3], up[3], pos[3];
+    float mat_inv[9];
+    float tan_fov;                /* tan(radians(fov * 0.5)) */
+    float aspect;
+    float lens_radius;
+    float focal_dist;
+    int32_t film_width, film_height;
+} zgl_camera;
+
+/* Sets the camera; (re)allocates and clears the frame when the film size changes. */
+zgl_status zgl_set_camera(zgl_context* ctx, const zgl_camera* camera);
+/* Per-pixel seed texture (uNoiseTex, RG32F width x height, src/core/Sampler.cpp:66-80). */
+zgl_status zgl_set_noise(zgl_context* ctx, const float* noise_rg, int32_t width, int32_t height);
+
+/* Closest hit (bvhHit, src/shader/intersection.glsl:395-427) for rays in SoA layout
+ * ox[n] oy[n] oz[n] dx[n] dy[n] dz[n].  prim = -1 and t = 1e8 on a miss; bary_bin =
+ * 16*min(15,int(16u)) + min(15,int(16v)) of the hit's barycentrics (0xffffffff on miss).
+ * Host pointers; synchronous. */
+zgl_status zgl_trace_closest(zgl_context* ctx, const float* rays, int64_t n, int32_t* prim, float* t,
+                             uint32_t* bary_bin);
+/* Same on device pointers, enqueued on the context stream (asynchronous). */
+zgl_status zgl_trace_closest_device(zgl_context* ctx, const float* d_rays, int64_t n, int32_t* d_prim,
+                                    float* d_t, uint32_t* d_bary_bin);
+/* zgl_trace_closest through the counting kernel variant: same outputs, and adds the
+ * algorithmic bytes (36 per row visited + 48 per leaf test) to the counters. */
+zgl_status zgl_trace_closest_counted(zgl_context* ctx, const float* rays, int64_t n, int32_t* prim, float* t,
+                                     uint32_t* bary_bin);
+/* Any hit (bvhTest, intersection.glsl:367-393) with per-ray t limits; host pointers. */
+zgl_status zgl_trace_any(zgl_context* ctx, const float* rays, const float* tmax, int64_t n, uint8_t* occluded);
+
+/* One or more progressive passes of an integrator (Integrator::renderOnePass). */
+typedef struct zgl_pass_params {
+    int32_t integrator;           /* ZGL_INTEGRATOR_* */
+    int32_t first_pass;           /* uSpp of the first pass (mCurSample) */
+    int32_t n_passes;             /* consecutive passes in this call */
+    int32_t free_counter;         /* uFreeCounter of the first pass (mFreeCounter) */
+    int32_t max_depth;            /* uMaxDepth */
+    int32_t russian_roulette;     /* uRussianRoulette */
+    int32_t sample_light;         /* uSampleLight (naive only) */
+    int32_t light_env_uniform;    /* uLightEnvUniformSample */
+    float light_portion;          /* uLightSamplePortion */
+    int32_t blocks_one_pass;      /* light: threadBlocksOnePass; triple: LPTBlocksOnePass */
+    int32_t loops_per_pass;       /* triple: LPTLoopsPerPass */
+    int32_t count_bytes;          /* accumulate algorithmic-byte counters (slower) */
+} zgl_pass_params;
+
+zgl_status zgl_render(zgl_context* ctx, const zgl_pass_params* params);
+/* Zero the frame (RGBA32F) and the light-splat film (planar R32F, 3W x H). */
+zgl_status zgl_clear_frame(zgl_context* ctx);
+/* Copy the RGBA32F frame (W*H*4 floats; the running sum of the naive integrator,
+ * the film copy for light/triple). Synchronous. */
+zgl_status zgl_read_frame(zgl_context* ctx, float* rgba, int64_t n_floats);
+/* Copy the planar film (3W x H floats). Synchronous. */
+zgl_status zgl_read_film(zgl_context* ctx, float* film, int64_t n_floats);
+/* Restore a frame (checkpoint resume). */
+zgl_status zgl_write_frame(zgl_context* ctx, const float* rgba, int64_t n_floats);
+/* Device pointer of the RGBA32F frame (valid until the film size changes). */
+void* zgl_frame_device_ptr(zgl_context* ctx);
+/* Counters: [0] algorithmic bytes, [1] MTBVH rows visited, [2] leaf triangle tests,
+ * [3] traversals. */
+zgl_status zgl_read_counters(zgl_context* ctx, uint64_t* out, int32_t n);
+zgl_status zgl_reset_counters(zgl_context* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ZGL_H */

@@ -1,0 +1,239 @@
+"""ctypes wrapper of the CPU oracle (oracle/liborc.so).  TEST INFRASTRUCTURE.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this, as the
+checker; the product never imports it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liborc.so")
+
+_fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int32)
+_up = C.POINTER(C.c_uint32)
+_u64 = C.POINTER(C.c_uint64)
+
+
+class OrcScene(C.Structure):
+    _fields_ = [
+        ("vertices", _fp), ("normals", _fp), ("n_vertices", C.c_int64),
+        ("texcoords", _fp), ("n_texcoords", C.c_int64),
+        ("indices", _ip), ("n_triangles", C.c_int64),
+        ("bounds", _fp), ("hit_table", _ip), ("bvh_size", C.c_int32),
+        ("mat_tex_indices", _ip), ("obj_prim_count", C.c_int32),
+        ("materials", _fp), ("n_materials", C.c_int32),
+        ("light_power", _fp), ("light_alias", _ip), ("light_prob", _fp),
+        ("n_light_triangles", C.c_int32), ("light_sum", C.c_float),
+        ("env_rgb16f", C.POINTER(C.c_uint16)), ("env_width", C.c_int32), ("env_height", C.c_int32),
+        ("env_alias", _ip), ("env_prob", _fp), ("env_sum", C.c_float), ("env_rotation", C.c_float),
+        ("sampler", C.c_int32),
+        ("cam_front", C.c_float * 3), ("cam_right", C.c_float * 3), ("cam_up", C.c_float * 3),
+        ("cam_pos", C.c_float * 3), ("cam_mat_inv", C.c_float * 9),
+        ("tan_fov", C.c_float), ("aspect", C.c_float), ("lens_radius", C.c_float), ("focal_dist", C.c_float),
+        ("film_width", C.c_int32), ("film_height", C.c_int32),
+        ("noise", _fp), ("noise_width", C.c_int32), ("noise_height", C.c_int32),
+    ]
+
+
+class OrcParams(C.Structure):
+    _fields_ = [
+        ("integrator", C.c_int32), ("first_pass", C.c_int32), ("n_passes", C.c_int32),
+        ("free_counter", C.c_int32), ("max_depth", C.c_int32), ("russian_roulette", C.c_int32),
+        ("sample_light", C.c_int32), ("light_env_uniform", C.c_int32), ("light_portion", C.c_float),
+        ("blocks_one_pass", C.c_int32), ("loops_per_pass", C.c_int32),
+    ]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        L.orc_build_mtbvh.argtypes = [_fp, C.c_int64, _up, C.c_int64, _fp, _ip]
+        L.orc_alias_build.argtypes = [_fp, C.c_int32, _ip, _fp]
+        L.orc_env_alias_build.argtypes = [_fp, C.c_int32, C.c_int32, _ip, _fp]
+        L.orc_env_alias_build.restype = C.c_float
+        L.orc_sobol_matrices.argtypes = [_up]
+        L.orc_sobol_matrices.restype = None
+        L.orc_sobol_sample.argtypes = [C.c_uint32, C.c_int32]
+        L.orc_sobol_sample.restype = C.c_uint32
+        L.orc_hash.argtypes = [C.c_uint32]
+        L.orc_hash.restype = C.c_uint32
+        L.orc_fmath.argtypes = [C.c_int, _fp, _fp, _fp, C.c_int64]
+        L.orc_fmath.restype = None
+        L.orc_trace_closest.argtypes = [C.POINTER(OrcScene), _fp, C.c_int64, _ip, _fp, _up, _u64, C.c_int]
+        L.orc_render_pixels.argtypes = [C.POINTER(OrcScene), C.POINTER(OrcParams), C.c_int, C.c_int, C.c_int,
+                                        C.c_int, _fp, _u64, C.c_int, C.c_int]
+        L.orc_render_light.argtypes = [C.POINTER(OrcScene), C.POINTER(OrcParams), C.c_int64, C.c_int64,
+                                       C.POINTER(C.c_double), _u64, C.c_int, C.c_int]
+        L.orc_bsdf_eval.argtypes = [_fp, _fp, _fp, _fp, C.c_int, _fp]
+        L.orc_bsdf_sample.argtypes = [_fp, _fp, _fp, C.c_int, _fp, C.c_uint32, _fp]
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def build_mtbvh(vertices, indices):
+    v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
+    i = np.ascontiguousarray(indices, np.uint32).reshape(-1, 3)
+    n = i.shape[0]
+    b = np.empty((2 * n - 1, 6), np.float32)
+    t = np.empty((6, 2 * n - 1, 3), np.int32)
+    assert lib().orc_build_mtbvh(_p(v, C.c_float), v.shape[0], _p(i, C.c_uint32), n, _p(b, C.c_float),
+                                 _p(t, C.c_int32)) == 0
+    return b, t
+
+
+def alias_build(pdf):
+    p = np.ascontiguousarray(pdf, np.float32)
+    a = np.zeros(len(p), np.int32)
+    q = np.zeros(len(p), np.float32)
+    lib().orc_alias_build(_p(p, C.c_float), len(p), _p(a, C.c_int32), _p(q, C.c_float))
+    return a, q
+
+
+def env_alias_build(rgb):
+    img = np.ascontiguousarray(rgb, np.float32)
+    h, w, _ = img.shape
+    a = np.zeros((h, w + 1), np.int32)
+    q = np.zeros((h, w + 1), np.float32)
+    s = lib().orc_env_alias_build(_p(img, C.c_float), w, h, _p(a, C.c_int32), _p(q, C.c_float))
+    return a, q, s
+
+
+def sobol_matrices():
+    out = np.zeros(256 * 32, np.uint32)
+    lib().orc_sobol_matrices(_p(out, C.c_uint32))
+    return out.reshape(256, 32)
+
+
+def hash_u32(x):
+    return int(lib().orc_hash(int(x) & 0xFFFFFFFF))
+
+
+def fmath(op, a, b=None):
+    ops = {"sin": 0, "cos": 1, "atan2": 2, "log": 3, "exp": 4, "pow": 5}
+    x = np.ascontiguousarray(a, np.float32)
+    y = np.ascontiguousarray(b if b is not None else np.zeros_like(x), np.float32)
+    out = np.empty_like(x)
+    lib().orc_fmath(ops[op], _p(x, C.c_float), _p(y, C.c_float), _p(out, C.c_float), x.size)
+    return out
+
+
+def scene_struct(ps):
+    """OrcScene view of a zillumgl_amd.scene.PackedScene (same buffers, reference layout)."""
+    s = OrcScene()
+    s.vertices = _p(ps.vertices, C.c_float)
+    s.normals = _p(ps.normals, C.c_float)
+    s.n_vertices = ps.vertices.shape[0]
+    s.texcoords = _p(ps.texcoords, C.c_float)
+    s.n_texcoords = ps.texcoords.shape[0]
+    s.indices = _p(ps.indices, C.c_int32)
+    s.n_triangles = ps.indices.shape[0]
+    s.bounds = _p(ps.bounds, C.c_float)
+    s.hit_table = _p(ps.hit_table, C.c_int32)
+    s.bvh_size = ps.bounds.shape[0]
+    s.mat_tex_indices = _p(ps.mat_tex_indices, C.c_int32)
+    s.obj_prim_count = ps.obj_prim_count
+    s.materials = _p(ps.materials, C.c_float)
+    s.n_materials = ps.materials.shape[0]
+    s.light_power = _p(ps.light_power, C.c_float)
+    s.light_alias = _p(ps.light_alias, C.c_int32)
+    s.light_prob = _p(ps.light_prob, C.c_float)
+    s.n_light_triangles = ps.n_light_triangles
+    s.light_sum = ps.light_sum
+    s.env_rgb16f = _p(ps.env_half, C.c_uint16)
+    s.env_width, s.env_height = ps.env_width, ps.env_height
+    s.env_alias = _p(ps.env_alias, C.c_int32)
+    s.env_prob = _p(ps.env_prob, C.c_float)
+    s.env_sum = ps.env_sum
+    s.env_rotation = ps.env_rotation
+    s.sampler = ps.sampler
+    cam = ps.camera
+    s.cam_front[:] = list(cam.front)
+    s.cam_right[:] = list(cam.right)
+    s.cam_up[:] = list(cam.up)
+    s.cam_pos[:] = list(cam.pos)
+    s.cam_mat_inv[:] = list(cam.mat_inv)
+    s.tan_fov, s.aspect, s.lens_radius, s.focal_dist = cam.tan_fov, cam.aspect, cam.lens_radius, cam.focal_dist
+    s.film_width, s.film_height = cam.film_width, cam.film_height
+    s.noise = _p(ps.noise, C.c_float)
+    s.noise_width, s.noise_height = ps.noise.shape[1], ps.noise.shape[0]
+    return s
+
+
+def trace_closest(ps, rays, nthreads=0):
+    r = np.ascontiguousarray(rays, np.float32)
+    n = r.size // 6
+    prim = np.empty(n, np.int32)
+    t = np.empty(n, np.float32)
+    b = np.empty(n, np.uint32)
+    cnt = np.zeros(4, np.uint64)
+    s = scene_struct(ps)
+    lib().orc_trace_closest(C.byref(s), _p(r, C.c_float), n, _p(prim, C.c_int32), _p(t, C.c_float),
+                            _p(b, C.c_uint32), _p(cnt, C.c_uint64), nthreads)
+    return prim, t, b, cnt
+
+
+def params(integrator=0, first_pass=0, n_passes=1, free_counter=1, max_depth=4, russian_roulette=False,
+           sample_light=True, light_env_uniform=False, light_portion=0.5, blocks_one_pass=32, loops_per_pass=1):
+    return OrcParams(integrator, first_pass, n_passes, free_counter, max_depth, int(russian_roulette),
+                     int(sample_light), int(light_env_uniform), light_portion, blocks_one_pass, loops_per_pass)
+
+
+def render_pixels(ps, prm, x0=0, y0=0, w=None, h=None, nthreads=0, fp64=False):
+    """Per-pixel sums of the camera-path integrators; returns ((h, w, 3) float32, counters)."""
+    w = ps.width if w is None else w
+    h = ps.height if h is None else h
+    out = np.zeros((h, w, 3), np.float32)
+    cnt = np.zeros(4, np.uint64)
+    s = scene_struct(ps)
+    assert lib().orc_render_pixels(C.byref(s), C.byref(prm), x0, y0, w, h, _p(out, C.c_float),
+                                   _p(cnt, C.c_uint64), nthreads, int(fp64)) == 0
+    return out, cnt
+
+
+def render_light(ps, prm, path_begin=0, path_end=None, nthreads=0, fp64=False):
+    """Splat film (h, w, 3) in float64 for the light-path integrators."""
+    if path_end is None:
+        path_end = 1536 * prm.blocks_one_pass * prm.n_passes
+    film = np.zeros((ps.height, ps.width, 3), np.float64)
+    cnt = np.zeros(4, np.uint64)
+    s = scene_struct(ps)
+    assert lib().orc_render_light(C.byref(s), C.byref(prm), path_begin, path_end,
+                                  film.ctypes.data_as(C.POINTER(C.c_double)), _p(cnt, C.c_uint64), nthreads,
+                                  int(fp64)) == 0
+    return film, cnt
+
+
+def bsdf_eval(mat_record, wo, wi, n, mode=0):
+    m = np.ascontiguousarray(mat_record, np.float32)
+    a = [np.ascontiguousarray(x, np.float32) for x in (wo, wi, n)]
+    out = np.zeros(4, np.float32)
+    lib().orc_bsdf_eval(_p(m, C.c_float), _p(a[0], C.c_float), _p(a[1], C.c_float), _p(a[2], C.c_float), mode,
+                        _p(out, C.c_float))
+    return out[:3], float(out[3])
+
+
+def bsdf_sample(mat_record, wo, n, u, mode=0, seed=1):
+    m = np.ascontiguousarray(mat_record, np.float32)
+    a = [np.ascontiguousarray(x, np.float32) for x in (wo, n, u)]
+    out = np.zeros(8, np.float32)
+    lib().orc_bsdf_sample(_p(m, C.c_float), _p(a[0], C.c_float), _p(a[1], C.c_float), mode, _p(a[2], C.c_float),
+                          seed, _p(out, C.c_float))
+    return out[:3], float(out[3]), out[4:7], int(out[7:8].view(np.uint32)[0])

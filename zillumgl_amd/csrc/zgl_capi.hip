@@ -1,0 +1,592 @@
+// zgl_capi.hip -- the C ABI: context, validated scene upload, pass dispatch.
+#include <cstring>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/zgl.h"
+#include "zgl_build.h"
+#include "zgl_internal.h"
+
+using namespace zgl;
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+};
+
+struct ZglError {
+    zgl_status code;
+    std::string msg;
+};
+
+void hipCheck(hipError_t e, const char* what)
+{
+    if (e == hipSuccess) return;
+    if (e == hipErrorOutOfMemory) throw ZglError{ZGL_E_OOM, std::string(what) + ": out of device memory"};
+    throw ZglError{ZGL_E_HIP, std::string(what) + ": " + hipGetErrorString(e)};
+}
+
+void invalid(const std::string& m) { throw ZglError{ZGL_E_INVALID, m}; }
+
+}  // namespace
+
+struct zgl_context {
+    int device = 0;
+    hipStream_t own = nullptr;
+    hipStream_t stream = nullptr;
+    std::string err;
+    DevBuf rows, tris, indices, vertices, normals, texcoords, matTex, materials;
+    DevBuf lightPower, lightAlias, lightProb, env, envAlias, envProb, noise, sobolM, sobolTab;
+    DevBuf frame, film, counters, scratchRays, scratchOut;
+    DevScene scene = {};
+    bool hasScene = false, hasCamera = false, hasNoise = false;
+    int filmW = 0, filmH = 0;
+
+    void upload(DevBuf& b, const void* src, size_t bytes, const char* what)
+    {
+        if (b.bytes < bytes || bytes == 0) {
+            b.release();
+            size_t alloc = bytes ? bytes : 16;
+            hipCheck(hipMalloc(&b.p, alloc), what);
+            b.bytes = alloc;
+        }
+        if (bytes) hipCheck(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, stream), what);
+    }
+    void ensure(DevBuf& b, size_t bytes, const char* what)
+    {
+        if (b.bytes >= bytes && b.p) return;
+        b.release();
+        hipCheck(hipMalloc(&b.p, bytes ? bytes : 16), what);
+        b.bytes = bytes ? bytes : 16;
+    }
+    void releaseAll()
+    {
+        DevBuf* all[] = {&rows, &tris, &indices, &vertices, &normals, &texcoords, &matTex, &materials,
+                         &lightPower, &lightAlias, &lightProb, &env, &envAlias, &envProb, &noise, &sobolM,
+                         &sobolTab, &frame, &film, &counters, &scratchRays, &scratchOut};
+        for (DevBuf* b : all) b->release();
+    }
+};
+
+namespace {
+
+template <class F> zgl_status guarded(zgl_context* ctx, F&& f)
+{
+    if (!ctx) return ZGL_E_INVALID;
+    try {
+        hipCheck(hipSetDevice(ctx->device), "hipSetDevice");
+        f();
+        ctx->err.clear();
+        return ZGL_OK;
+    } catch (const ZglError& e) {
+        ctx->err = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        ctx->err = "host allocation failed";
+        return ZGL_E_OOM;
+    } catch (const std::exception& e) {
+        ctx->err = e.what();
+        return ZGL_E_INVALID;
+    }
+}
+
+template <class F> zgl_status guardedHost(F&& f)
+{
+    try {
+        f();
+        return ZGL_OK;
+    } catch (const ZglError& e) {
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        return ZGL_E_OOM;
+    } catch (...) {
+        return ZGL_E_INVALID;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---------------------------------------------------------------- host producers
+zgl_status zgl_build_mtbvh(const float* vertices, int64_t n_vertices, const uint32_t* indices, int64_t n_triangles,
+                           float* bounds_out, int32_t* hit_table_out)
+{
+    if (!vertices || !indices || !bounds_out || !hit_table_out || n_triangles < 1 || n_vertices < 1 ||
+        n_triangles > (1LL << 29))
+        return ZGL_E_INVALID;
+    for (int64_t i = 0; i < n_triangles * 3; i++)
+        if ((int64_t)indices[i] >= n_vertices) return ZGL_E_INVALID;
+    return guardedHost([&] {
+        std::vector<float> b;
+        std::vector<int32_t> h;
+        buildMtbvh(vertices, indices, n_triangles, b, h);
+        std::memcpy(bounds_out, b.data(), b.size() * sizeof(float));
+        std::memcpy(hit_table_out, h.data(), h.size() * sizeof(int32_t));
+    });
+}
+
+zgl_status zgl_alias_build(const float* pdf, int32_t n, int32_t* alias_out, float* prob_out)
+{
+    if (n < 0 || (n > 0 && (!pdf || !alias_out || !prob_out))) return ZGL_E_INVALID;
+    return guardedHost([&] { aliasBuild(pdf, n, alias_out, prob_out); });
+}
+
+zgl_status zgl_env_alias_build(const float* rgb, int32_t width, int32_t height, int32_t* alias_out,
+                               float* prob_out, float* sum_out)
+{
+    if (!rgb || !alias_out || !prob_out || !sum_out || width < 1 || height < 1) return ZGL_E_INVALID;
+    return guardedHost([&] { *sum_out = envAliasBuild(rgb, width, height, alias_out, prob_out); });
+}
+
+uint32_t zgl_sobol_sample(uint32_t index, int32_t dim, uint32_t scramble)
+{
+    if (dim < 0 || dim >= 256) return 0;
+    return sobolSample(index, dim, scramble);
+}
+
+void zgl_sobol_matrices(uint32_t* out)
+{
+    if (out) std::memcpy(out, sobolMatrices(), 256 * 32 * sizeof(uint32_t));
+}
+
+void zgl_float_to_half(const float* in, uint16_t* out, int64_t n)
+{
+    if (!in || !out) return;
+    for (int64_t i = 0; i < n; i++) out[i] = floatToHalf(in[i]);
+}
+
+// ---------------------------------------------------------------- context
+int32_t zgl_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+zgl_status zgl_create(int32_t device, zgl_context** out)
+{
+    if (!out) return ZGL_E_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return ZGL_E_NODEVICE;
+    if (device < 0 || device >= n) return ZGL_E_INVALID;
+    zgl_context* ctx = new (std::nothrow) zgl_context();
+    if (!ctx) return ZGL_E_OOM;
+    ctx->device = device;
+    zgl_status st = guarded(ctx, [&] {
+        hipCheck(hipStreamCreateWithFlags(&ctx->own, hipStreamNonBlocking), "hipStreamCreate");
+        ctx->stream = ctx->own;
+        ctx->upload(ctx->sobolM, sobolMatrices(), 256 * 32 * sizeof(uint32_t), "sobol matrices");
+        ctx->ensure(ctx->counters, NumCounters * sizeof(unsigned long long), "counters");
+        hipCheck(hipMemsetAsync(ctx->counters.p, 0, NumCounters * sizeof(unsigned long long), ctx->stream),
+                 "counters");
+        hipCheck(hipStreamSynchronize(ctx->stream), "sync");
+    });
+    if (st != ZGL_OK) {
+        ctx->releaseAll();
+        if (ctx->own) (void)hipStreamDestroy(ctx->own);
+        delete ctx;
+        return st;
+    }
+    *out = ctx;
+    return ZGL_OK;
+}
+
+void zgl_destroy(zgl_context* ctx)
+{
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    ctx->releaseAll();
+    if (ctx->own) (void)hipStreamDestroy(ctx->own);
+    delete ctx;
+}
+
+const char* zgl_last_error(const zgl_context* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+zgl_status zgl_set_stream(zgl_context* ctx, void* s)
+{
+    return guarded(ctx, [&] {
+        hipCheck(hipStreamSynchronize(ctx->stream), "sync");
+        ctx->stream = s ? (hipStream_t)s : ctx->own;
+    });
+}
+
+void* zgl_get_stream(zgl_context* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+zgl_status zgl_synchronize(zgl_context* ctx)
+{
+    return guarded(ctx, [&] { hipCheck(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize"); });
+}
+
+// ---------------------------------------------------------------- scene upload
+zgl_status zgl_upload_scene(zgl_context* ctx, const zgl_scene_buffers* sb)
+{
+    return guarded(ctx, [&] {
+        if (!sb) invalid("null scene");
+        const int64_t nV = sb->n_vertices, nT = sb->n_triangles;
+        if (nV < 3 || nT < 1 || !sb->vertices || !sb->normals || !sb->indices) invalid("empty geometry");
+        if (nT > (1LL << 29) || nV > (1LL << 30)) invalid("scene too large");
+        if (sb->bvh_size != 2 * nT - 1 || !sb->bounds || !sb->hit_table) invalid("bvh_size must be 2*n_triangles-1");
+        if (sb->obj_prim_count < 0 || sb->obj_prim_count > nT) invalid("obj_prim_count out of range");
+        if (sb->n_light_triangles != nT - sb->obj_prim_count) invalid("n_light_triangles != n_triangles - obj_prim_count");
+        if (sb->n_texcoords < 0 || sb->n_texcoords > nV || (sb->n_texcoords > 0 && !sb->texcoords))
+            invalid("texcoords");
+        if (sb->n_materials < 1 || !sb->materials) invalid("materials");
+        if (sb->obj_prim_count > 0 && !sb->mat_tex_indices) invalid("mat_tex_indices");
+        if (sb->env_width < 1 || sb->env_height < 1 || !sb->env_rgb16f || !sb->env_alias || !sb->env_prob)
+            invalid("environment map");
+        for (int64_t i = 0; i < 3 * nT; i++)
+            if (sb->indices[i] < 0 || sb->indices[i] >= nV) invalid("index out of range");
+        for (int64_t i = 0; i < sb->obj_prim_count; i++)
+            if ((sb->mat_tex_indices[i] & 0xffff) >= sb->n_materials) invalid("material index out of range");
+        const int64_t B = sb->bvh_size;
+        // Every row must point forward so that every traversal terminates.
+        for (int t = 0; t < 6; t++)
+            for (int64_t k = 0; k < B; k++) {
+                const int32_t* r = sb->hit_table + (t * B + k) * 3;
+                if (r[0] < 0 || r[0] >= B || r[1] < -1 || r[1] >= nT || r[2] <= k || r[2] > B)
+                    invalid("malformed hit table row");
+            }
+        const int nL = sb->n_light_triangles;
+        if (nL > 0) {
+            if (!sb->light_power || !sb->light_alias || !sb->light_prob) invalid("light tables");
+            for (int i = 0; i < nL; i++)
+                if (sb->light_alias[i] < 0 || sb->light_alias[i] >= nL) invalid("light alias out of range");
+        }
+        const int w = sb->env_width, h = sb->env_height;
+        for (int i = 0; i < h; i++) {
+            for (int j = 0; j < w; j++) {
+                int a = sb->env_alias[(size_t)i * (w + 1) + j];
+                if (a < 0 || a >= w) invalid("env alias out of range");
+            }
+            int a = sb->env_alias[(size_t)i * (w + 1) + w];
+            if (a < 0 || a >= h) invalid("env marginal alias out of range");
+        }
+
+        // MTBVH row records: {pMin.xyz, pMax.x} {pMax.yz, prim, miss}
+        std::vector<float> rows((size_t)6 * B * 8);
+        for (int t = 0; t < 6; t++)
+            for (int64_t k = 0; k < B; k++) {
+                const int32_t* r = sb->hit_table + (t * B + k) * 3;
+                const float* bb = sb->bounds + (size_t)r[0] * 6;
+                float* o = rows.data() + ((size_t)t * B + k) * 8;
+                o[0] = bb[0]; o[1] = bb[1]; o[2] = bb[2]; o[3] = bb[3];
+                o[4] = bb[4]; o[5] = bb[5];
+                std::memcpy(o + 6, &r[1], 4);
+                std::memcpy(o + 7, &r[2], 4);
+            }
+        std::vector<float> tris((size_t)nT * 12, 0.0f);
+        for (int64_t i = 0; i < nT; i++) {
+            float* o = tris.data() + i * 12;
+            for (int v = 0; v < 3; v++)
+                for (int c = 0; c < 3; c++) o[v * 3 + c] = sb->vertices[(size_t)sb->indices[3 * i + v] * 3 + c];
+        }
+        std::vector<uint16_t> env((size_t)w * h * 4, 0);
+        for (size_t i = 0; i < (size_t)w * h; i++)
+            for (int c = 0; c < 3; c++) env[i * 4 + c] = sb->env_rgb16f[i * 3 + c];
+
+        ctx->upload(ctx->rows, rows.data(), rows.size() * 4, "rows");
+        ctx->upload(ctx->tris, tris.data(), tris.size() * 4, "triangles");
+        ctx->upload(ctx->indices, sb->indices, (size_t)nT * 12, "indices");
+        ctx->upload(ctx->vertices, sb->vertices, (size_t)nV * 12, "vertices");
+        ctx->upload(ctx->normals, sb->normals, (size_t)nV * 12, "normals");
+        ctx->upload(ctx->texcoords, sb->texcoords, (size_t)sb->n_texcoords * 8, "texcoords");
+        ctx->upload(ctx->matTex, sb->mat_tex_indices, (size_t)sb->obj_prim_count * 4, "matTexIndices");
+        ctx->upload(ctx->materials, sb->materials, (size_t)sb->n_materials * 64, "materials");
+        ctx->upload(ctx->lightPower, sb->light_power, (size_t)nL * 12, "lightPower");
+        ctx->upload(ctx->lightAlias, sb->light_alias, (size_t)nL * 4, "lightAlias");
+        ctx->upload(ctx->lightProb, sb->light_prob, (size_t)nL * 4, "lightProb");
+        ctx->upload(ctx->env, env.data(), env.size() * 2, "env");
+        ctx->upload(ctx->envAlias, sb->env_alias, (size_t)(w + 1) * h * 4, "envAlias");
+        ctx->upload(ctx->envProb, sb->env_prob, (size_t)(w + 1) * h * 4, "envProb");
+        hipCheck(hipStreamSynchronize(ctx->stream), "scene upload");
+
+        DevScene& s = ctx->scene;
+        s.rows = (const float4*)ctx->rows.p;
+        s.tris = (const float4*)ctx->tris.p;
+        s.indices = (const int32_t*)ctx->indices.p;
+        s.vertices = (const float*)ctx->vertices.p;
+        s.normals = (const float*)ctx->normals.p;
+        s.texcoords = (const float*)ctx->texcoords.p;
+        s.matTexIndices = (const int32_t*)ctx->matTex.p;
+        s.materials = (const float4*)ctx->materials.p;
+        s.lightPower = (const float*)ctx->lightPower.p;
+        s.lightAlias = (const int32_t*)ctx->lightAlias.p;
+        s.lightProb = (const float*)ctx->lightProb.p;
+        s.env = (const ushort4*)ctx->env.p;
+        s.envAlias = (const int32_t*)ctx->envAlias.p;
+        s.envProb = (const float*)ctx->envProb.p;
+        s.sobolM = (const uint32_t*)ctx->sobolM.p;
+        s.nVertices = nV;
+        s.nTexcoords = (int32_t)sb->n_texcoords;
+        s.nTriangles = (int32_t)nT;
+        s.objPrimCount = sb->obj_prim_count;
+        s.bvhSize = sb->bvh_size;
+        s.nMaterials = sb->n_materials;
+        s.nLightTriangles = nL;
+        s.lightSum = sb->light_sum;
+        s.envW = w;
+        s.envH = h;
+        s.envSum = sb->env_sum;
+        sincosZ(sb->env_rotation, &s.envRotSin, &s.envRotCos);
+        sincosZ(-sb->env_rotation, &s.envRotSinN, &s.envRotCosN);
+        s.sampler = sb->sampler ? 1 : 0;
+        ctx->hasScene = true;
+    });
+}
+
+zgl_status zgl_set_camera(zgl_context* ctx, const zgl_camera* c)
+{
+    return guarded(ctx, [&] {
+        if (!c) invalid("null camera");
+        if (c->film_width < 1 || c->film_height < 1 || c->film_width > 16384 || c->film_height > 16384)
+            invalid("film size");
+        DevScene& s = ctx->scene;
+        s.camF = v3(c->front[0], c->front[1], c->front[2]);
+        s.camR = v3(c->right[0], c->right[1], c->right[2]);
+        s.camU = v3(c->up[0], c->up[1], c->up[2]);
+        s.camPos = v3(c->pos[0], c->pos[1], c->pos[2]);
+        s.camMatInv.c0 = v3(c->mat_inv[0], c->mat_inv[1], c->mat_inv[2]);
+        s.camMatInv.c1 = v3(c->mat_inv[3], c->mat_inv[4], c->mat_inv[5]);
+        s.camMatInv.c2 = v3(c->mat_inv[6], c->mat_inv[7], c->mat_inv[8]);
+        s.tanFOV = c->tan_fov;
+        s.camAsp = c->aspect;
+        s.lensRadius = c->lens_radius;
+        s.focalDist = c->focal_dist;
+        if (c->film_width != ctx->filmW || c->film_height != ctx->filmH || !ctx->frame.p) {
+            const size_t px = (size_t)c->film_width * c->film_height;
+            ctx->frame.release();
+            ctx->film.release();
+            ctx->ensure(ctx->frame, px * 16, "frame");
+            ctx->ensure(ctx->film, px * 12, "film");
+            hipCheck(launchFill((float*)ctx->frame.p, (int64_t)px * 4, 0.0f, ctx->stream), "clear");
+            hipCheck(launchFill((float*)ctx->film.p, (int64_t)px * 3, 0.0f, ctx->stream), "clear");
+            ctx->filmW = c->film_width;
+            ctx->filmH = c->film_height;
+        }
+        s.filmW = ctx->filmW;
+        s.filmH = ctx->filmH;
+        ctx->hasCamera = true;
+    });
+}
+
+zgl_status zgl_set_noise(zgl_context* ctx, const float* noise, int32_t w, int32_t h)
+{
+    return guarded(ctx, [&] {
+        if (!noise || w < 1 || h < 1 || (int64_t)w * h > (1LL << 28)) invalid("noise texture");
+        ctx->upload(ctx->noise, noise, (size_t)w * h * 8, "noise");
+        hipCheck(hipStreamSynchronize(ctx->stream), "noise upload");
+        ctx->scene.noise = (const float2*)ctx->noise.p;
+        ctx->scene.noiseW = w;
+        ctx->scene.noiseH = h;
+        ctx->hasNoise = true;
+    });
+}
+
+// ---------------------------------------------------------------- tracing
+static void checkRays(zgl_context* ctx, int64_t n)
+{
+    if (!ctx->hasScene) throw ZglError{ZGL_E_STATE, "no scene uploaded"};
+    if (n < 0 || n > (1LL << 31)) invalid("ray count");
+}
+
+zgl_status zgl_trace_closest_device(zgl_context* ctx, const float* d_rays, int64_t n, int32_t* d_prim, float* d_t,
+                                    uint32_t* d_bin)
+{
+    return guarded(ctx, [&] {
+        checkRays(ctx, n);
+        if (n && (!d_rays || !d_prim || !d_t || !d_bin)) invalid("null device pointer");
+        hipCheck(launchTraceClosest(ctx->scene, d_rays, n, d_prim, d_t, d_bin, nullptr, ctx->stream),
+                 "trace_closest");
+    });
+}
+
+static zgl_status traceClosestHost(zgl_context* ctx, const float* rays, int64_t n, int32_t* prim, float* t,
+                                   uint32_t* bin, bool count)
+{
+    return guarded(ctx, [&] {
+        checkRays(ctx, n);
+        if (n == 0) return;
+        if (!rays || !prim || !t || !bin) invalid("null pointer");
+        ctx->upload(ctx->scratchRays, rays, (size_t)n * 24, "rays");
+        ctx->ensure(ctx->scratchOut, (size_t)n * 12, "trace output");
+        int32_t* dp = (int32_t*)ctx->scratchOut.p;
+        float* dt = (float*)(dp + n);
+        uint32_t* db = (uint32_t*)(dt + n);
+        hipCheck(launchTraceClosest(ctx->scene, (const float*)ctx->scratchRays.p, n, dp, dt, db,
+                                    count ? (unsigned long long*)ctx->counters.p : nullptr, ctx->stream),
+                 "trace_closest");
+        hipCheck(hipMemcpyAsync(prim, dp, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream), "copy");
+        hipCheck(hipMemcpyAsync(t, dt, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream), "copy");
+        hipCheck(hipMemcpyAsync(bin, db, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream), "copy");
+        hipCheck(hipStreamSynchronize(ctx->stream), "trace_closest");
+    });
+}
+
+zgl_status zgl_trace_closest(zgl_context* ctx, const float* rays, int64_t n, int32_t* prim, float* t,
+                             uint32_t* bin)
+{
+    return traceClosestHost(ctx, rays, n, prim, t, bin, false);
+}
+
+zgl_status zgl_trace_closest_counted(zgl_context* ctx, const float* rays, int64_t n, int32_t* prim, float* t,
+                                     uint32_t* bin)
+{
+    return traceClosestHost(ctx, rays, n, prim, t, bin, true);
+}
+
+zgl_status zgl_trace_any(zgl_context* ctx, const float* rays, const float* tmax, int64_t n, uint8_t* occ)
+{
+    return guarded(ctx, [&] {
+        checkRays(ctx, n);
+        if (n == 0) return;
+        if (!rays || !tmax || !occ) invalid("null pointer");
+        std::vector<float> buf((size_t)n * 7);
+        std::memcpy(buf.data(), rays, (size_t)n * 24);
+        std::memcpy(buf.data() + 6 * n, tmax, (size_t)n * 4);
+        ctx->upload(ctx->scratchRays, buf.data(), buf.size() * 4, "rays");
+        ctx->ensure(ctx->scratchOut, (size_t)n, "trace output");
+        const float* dr = (const float*)ctx->scratchRays.p;
+        hipCheck(launchTraceAny(ctx->scene, dr, dr + 6 * n, n, (uint8_t*)ctx->scratchOut.p, ctx->stream),
+                 "trace_any");
+        hipCheck(hipMemcpyAsync(occ, ctx->scratchOut.p, (size_t)n, hipMemcpyDeviceToHost, ctx->stream), "copy");
+        hipCheck(hipStreamSynchronize(ctx->stream), "trace_any");
+    });
+}
+
+// ---------------------------------------------------------------- rendering
+zgl_status zgl_render(zgl_context* ctx, const zgl_pass_params* p)
+{
+    return guarded(ctx, [&] {
+        if (!p) invalid("null params");
+        if (!ctx->hasScene || !ctx->hasCamera || !ctx->hasNoise)
+            throw ZglError{ZGL_E_STATE, "scene, camera and noise must be set before rendering"};
+        if (p->n_passes < 1 || p->n_passes > 65536 || p->first_pass < 0 || p->first_pass > (1 << 24) ||
+            p->max_depth < 0 || p->max_depth > 1024)
+            invalid("pass range / depth");
+        if (p->integrator < 0 || p->integrator > 2) invalid("integrator");
+        if ((p->integrator == ZGL_INTEGRATOR_LIGHT || p->integrator == ZGL_INTEGRATOR_TRIPLE) &&
+            ctx->scene.nLightTriangles < 1)
+            invalid("light and triple integrators need light triangles");
+        if (p->integrator != ZGL_INTEGRATOR_NAIVE && (p->blocks_one_pass < 1 || p->blocks_one_pass > 65536))
+            invalid("blocks_one_pass");
+        if (p->integrator == ZGL_INTEGRATOR_TRIPLE && (p->loops_per_pass < 1 || p->loops_per_pass > 1024))
+            invalid("loops_per_pass");
+        // Sobol points of the passes in this launch, plus one for dims spilling past 256.
+        const int first = p->first_pass, count = p->n_passes + 1;
+        ctx->ensure(ctx->sobolTab, (size_t)count * 256 * 4, "sobol table");
+        hipCheck(launchSobolTable((const uint32_t*)ctx->sobolM.p, (uint32_t*)ctx->sobolTab.p, first, count,
+                                  ctx->stream),
+                 "sobol table");
+        DevScene s = ctx->scene;
+        s.sobolTab = (const uint32_t*)ctx->sobolTab.p;
+        s.sobolTabFirst = first;
+        s.sobolTabCount = count;
+        PassParams pp;
+        pp.firstPass = p->first_pass;
+        pp.nPasses = p->n_passes;
+        pp.freeCounter = p->free_counter;
+        pp.maxDepth = p->max_depth;
+        pp.russianRoulette = p->russian_roulette;
+        pp.sampleLight = p->sample_light;
+        pp.lightEnvUniform = p->light_env_uniform;
+        pp.lightPortion = p->light_portion;
+        pp.loopsPerPass = p->loops_per_pass > 0 ? p->loops_per_pass : 1;
+        pp.pathsPerPass = 1536 * (p->blocks_one_pass > 0 ? p->blocks_one_pass : 1);
+        pp.lptScale = 0.0f;
+        unsigned long long* cnt = p->count_bytes ? (unsigned long long*)ctx->counters.p : nullptr;
+        float4* frame = (float4*)ctx->frame.p;
+        float* film = (float*)ctx->film.p;
+        if (p->integrator == ZGL_INTEGRATOR_NAIVE) {
+            hipCheck(launchNaive(s, pp, frame, cnt, ctx->stream), "naive pass");
+        } else if (p->integrator == ZGL_INTEGRATOR_LIGHT) {
+            s.sampler = 0;  // LightPath.cpp:49 forces the independent sampler
+            hipCheck(launchLight(s, pp, film, cnt, ctx->stream), "light pass");
+            hipCheck(launchFilmToFrame(film, frame, ctx->filmW, ctx->filmH, ctx->stream), "film copy");
+        } else {
+            // uScale = W*H / (blocks * loops * 1536)  (TriplePath.cpp:76-77)
+            pp.lptScale = (float)(ctx->filmW * ctx->filmH) / (float)(p->blocks_one_pass * pp.loopsPerPass * 1536);
+            hipCheck(launchTriplePT(s, pp, film, cnt, ctx->stream), "triple camera pass");
+            DevScene sl = s;
+            sl.sampler = 0;  // TriplePath.cpp:72
+            hipCheck(launchTripleLPT(sl, pp, film, cnt, ctx->stream), "triple light pass");
+            hipCheck(launchFilmToFrame(film, frame, ctx->filmW, ctx->filmH, ctx->stream), "film copy");
+        }
+    });
+}
+
+zgl_status zgl_clear_frame(zgl_context* ctx)
+{
+    return guarded(ctx, [&] {
+        if (!ctx->hasCamera) throw ZglError{ZGL_E_STATE, "no camera"};
+        const int64_t px = (int64_t)ctx->filmW * ctx->filmH;
+        hipCheck(launchFill((float*)ctx->frame.p, px * 4, 0.0f, ctx->stream), "clear");
+        hipCheck(launchFill((float*)ctx->film.p, px * 3, 0.0f, ctx->stream), "clear");
+    });
+}
+
+zgl_status zgl_read_frame(zgl_context* ctx, float* rgba, int64_t n)
+{
+    return guarded(ctx, [&] {
+        if (!ctx->hasCamera) throw ZglError{ZGL_E_STATE, "no camera"};
+        const int64_t need = (int64_t)ctx->filmW * ctx->filmH * 4;
+        if (!rgba || n < need) invalid("frame buffer too small");
+        hipCheck(hipMemcpyAsync(rgba, ctx->frame.p, (size_t)need * 4, hipMemcpyDeviceToHost, ctx->stream), "read");
+        hipCheck(hipStreamSynchronize(ctx->stream), "read frame");
+    });
+}
+
+zgl_status zgl_read_film(zgl_context* ctx, float* film, int64_t n)
+{
+    return guarded(ctx, [&] {
+        if (!ctx->hasCamera) throw ZglError{ZGL_E_STATE, "no camera"};
+        const int64_t need = (int64_t)ctx->filmW * ctx->filmH * 3;
+        if (!film || n < need) invalid("film buffer too small");
+        hipCheck(hipMemcpyAsync(film, ctx->film.p, (size_t)need * 4, hipMemcpyDeviceToHost, ctx->stream), "read");
+        hipCheck(hipStreamSynchronize(ctx->stream), "read film");
+    });
+}
+
+zgl_status zgl_write_frame(zgl_context* ctx, const float* rgba, int64_t n)
+{
+    return guarded(ctx, [&] {
+        if (!ctx->hasCamera) throw ZglError{ZGL_E_STATE, "no camera"};
+        const int64_t need = (int64_t)ctx->filmW * ctx->filmH * 4;
+        if (!rgba || n != need) invalid("frame size mismatch");
+        hipCheck(hipMemcpyAsync(ctx->frame.p, rgba, (size_t)need * 4, hipMemcpyHostToDevice, ctx->stream), "write");
+        hipCheck(hipStreamSynchronize(ctx->stream), "write frame");
+    });
+}
+
+void* zgl_frame_device_ptr(zgl_context* ctx) { return ctx ? ctx->frame.p : nullptr; }
+
+zgl_status zgl_read_counters(zgl_context* ctx, uint64_t* out, int32_t n)
+{
+    return guarded(ctx, [&] {
+        if (!out || n < 1 || n > NumCounters) invalid("counter buffer");
+        hipCheck(hipMemcpyAsync(out, ctx->counters.p, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream), "read");
+        hipCheck(hipStreamSynchronize(ctx->stream), "read counters");
+    });
+}
+
+zgl_status zgl_reset_counters(zgl_context* ctx)
+{
+    return guarded(ctx, [&] {
+        hipCheck(hipMemsetAsync(ctx->counters.p, 0, NumCounters * 8, ctx->stream), "reset counters");
+        hipCheck(hipStreamSynchronize(ctx->stream), "reset counters");
+    });
+}
+
+}  // extern "C"

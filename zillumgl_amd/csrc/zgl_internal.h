@@ -1,0 +1,48 @@
+// zgl_internal.h -- launch interface between the C ABI layer and the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "zgl_device.h"
+
+namespace zgl {
+
+// Integrator identifiers (the reference's NaivePath / LightPath / TriplePath).
+constexpr int IntegNaive = 0;
+constexpr int IntegLight = 1;
+constexpr int IntegTriple = 2;
+
+struct PassParams {
+    int firstPass;      // uSpp of the first pass in the launch
+    int nPasses;        // passes rendered by this launch
+    int freeCounter;    // uFreeCounter of the first pass (incremented per pass)
+    int maxDepth;
+    int russianRoulette;
+    int sampleLight;
+    int lightEnvUniform;
+    float lightPortion;
+    int pathsPerPass;   // light tracer: workgroup size 1536 x blocks per pass
+    int loopsPerPass;   // triple LPT loops per thread
+    float lptScale;     // triple LPT uScale
+};
+
+// Counter slots (uint64): 0 bytes, 1 rows, 2 leaf tests, 3 traversals, 4 samples
+constexpr int NumCounters = 8;
+
+hipError_t launchSobolTable(const uint32_t* sobolM, uint32_t* tab, int first, int count, hipStream_t st);
+hipError_t launchFill(float* p, int64_t n, float v, hipStream_t st);
+hipError_t launchFilmToFrame(const float* film, float4* frame, int w, int h, hipStream_t st);
+hipError_t launchTraceClosest(const DevScene& s, const float* rays, int64_t n, int32_t* prim, float* t,
+                              uint32_t* bin, unsigned long long* counters, hipStream_t st);
+hipError_t launchTraceAny(const DevScene& s, const float* rays, const float* tmax, int64_t n,
+                          uint8_t* occluded, hipStream_t st);
+hipError_t launchNaive(const DevScene& s, const PassParams& p, float4* frame, unsigned long long* counters,
+                       hipStream_t st);
+hipError_t launchLight(const DevScene& s, const PassParams& p, float* film, unsigned long long* counters,
+                       hipStream_t st);
+hipError_t launchTriplePT(const DevScene& s, const PassParams& p, float* film, unsigned long long* counters,
+                          hipStream_t st);
+hipError_t launchTripleLPT(const DevScene& s, const PassParams& p, float* film, unsigned long long* counters,
+                           hipStream_t st);
+
+}  // namespace zgl

@@ -1,0 +1,235 @@
+"""Device context and the progressive integrators over the C ABI.
+
+``NaivePathIntegrator``, ``LightPathIntegrator`` and ``TriplePathIntegrator`` keep
+the reference's Integrator interface (src/core/Integrator.h:25-52): init,
+renderOnePass, reset, getFrame, resultScale, with the same pass counters
+(mCurSample, mFreeCounter, samplePerPixel) and parameter defaults
+(Integrator.h:56-143).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, ptr
+
+
+class Context:
+    """One zgl_context (device buffers + one HIP stream)."""
+
+    def __init__(self, device=0):
+        L = _lib.lib()
+        h = C.c_void_p()
+        st = L.zgl_create(device, C.byref(h))
+        if st != _lib.ZGL_OK:
+            raise _lib.ZglError(st, f"zgl_create(device={device}) failed")
+        self.h = h
+        self.width = self.height = 0
+        self._keep = None
+
+    def close(self):
+        if self.h:
+            _lib.lib().zgl_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st):
+        check(st, self.h)
+
+    def upload(self, packed):
+        sb = packed.as_struct()
+        self._check(_lib.lib().zgl_upload_scene(self.h, C.byref(sb)))
+        self.set_camera(packed.camera)
+        self.set_noise(packed.noise)
+        return self
+
+    def set_camera(self, cam):
+        self._check(_lib.lib().zgl_set_camera(self.h, C.byref(cam)))
+        self.width, self.height = cam.film_width, cam.film_height
+
+    def set_noise(self, noise):
+        n = np.ascontiguousarray(noise, np.float32)
+        self._check(_lib.lib().zgl_set_noise(self.h, ptr(n, C.c_float), n.shape[1], n.shape[0]))
+
+    def set_stream(self, stream_handle):
+        self._check(_lib.lib().zgl_set_stream(self.h, C.c_void_p(stream_handle)))
+
+    def stream(self):
+        return _lib.lib().zgl_get_stream(self.h)
+
+    def synchronize(self):
+        self._check(_lib.lib().zgl_synchronize(self.h))
+
+    def trace_closest(self, rays, counted=False):
+        r = np.ascontiguousarray(rays, np.float32)
+        n = r.shape[-1] if r.ndim == 2 else r.size // 6
+        prim = np.empty(n, np.int32)
+        t = np.empty(n, np.float32)
+        b = np.empty(n, np.uint32)
+        f = _lib.lib().zgl_trace_closest_counted if counted else _lib.lib().zgl_trace_closest
+        self._check(f(self.h, ptr(r, C.c_float), n, ptr(prim, C.c_int32), ptr(t, C.c_float), ptr(b, C.c_uint32)))
+        return prim, t, b
+
+    def trace_closest_device(self, d_rays, n, d_prim, d_t, d_bin):
+        self._check(_lib.lib().zgl_trace_closest_device(self.h, C.c_void_p(d_rays), n, C.c_void_p(d_prim),
+                                                        C.c_void_p(d_t), C.c_void_p(d_bin)))
+
+    def trace_any(self, rays, tmax):
+        r = np.ascontiguousarray(rays, np.float32)
+        n = r.size // 6
+        tm = np.ascontiguousarray(tmax, np.float32)
+        occ = np.empty(n, np.uint8)
+        self._check(_lib.lib().zgl_trace_any(self.h, ptr(r, C.c_float), ptr(tm, C.c_float), n,
+                                             ptr(occ, C.c_uint8)))
+        return occ.astype(bool)
+
+    def render(self, integrator, first_pass, n_passes, free_counter, max_depth, russian_roulette=False,
+               sample_light=True, light_env_uniform=False, light_portion=0.5, blocks_one_pass=32,
+               loops_per_pass=1, count_bytes=False):
+        p = _lib.PassParams(integrator, first_pass, n_passes, free_counter, max_depth, int(russian_roulette),
+                            int(sample_light), int(light_env_uniform), light_portion, blocks_one_pass,
+                            loops_per_pass, int(count_bytes))
+        self._check(_lib.lib().zgl_render(self.h, C.byref(p)))
+
+    def clear(self):
+        self._check(_lib.lib().zgl_clear_frame(self.h))
+
+    def read_frame(self):
+        out = np.empty((self.height, self.width, 4), np.float32)
+        self._check(_lib.lib().zgl_read_frame(self.h, ptr(out, C.c_float), out.size))
+        return out
+
+    def read_film(self):
+        out = np.empty((self.height, self.width * 3), np.float32)
+        self._check(_lib.lib().zgl_read_film(self.h, ptr(out, C.c_float), out.size))
+        return out.reshape(self.height, self.width, 3)
+
+    def write_frame(self, rgba):
+        a = np.ascontiguousarray(rgba, np.float32)
+        self._check(_lib.lib().zgl_write_frame(self.h, ptr(a, C.c_float), a.size))
+
+    def frame_device_ptr(self):
+        return _lib.lib().zgl_frame_device_ptr(self.h)
+
+    def counters(self):
+        out = np.zeros(4, np.uint64)
+        self._check(_lib.lib().zgl_read_counters(self.h, ptr(out, C.c_uint64), 4))
+        return out
+
+    def reset_counters(self):
+        self._check(_lib.lib().zgl_reset_counters(self.h))
+
+
+class NaivePathIntegrator:
+    """Naive MIS path tracer (src/integrator/NaivePath.cpp:77-115); defaults from
+    PathIntegParam (src/core/Integrator.h:56-66)."""
+
+    def __init__(self, max_depth=4, russian_roulette=False, sample_light=True, light_env_uniform=False,
+                 light_portion=0.5):
+        self.max_depth = max_depth
+        self.russian_roulette = russian_roulette
+        self.sample_light = sample_light
+        self.light_env_uniform = light_env_uniform
+        self.light_portion = light_portion
+        self.cur_sample = 0
+        self.free_counter = 0
+
+    def init(self, ctx):
+        self.ctx = ctx
+        self.reset()
+
+    def reset(self):
+        self.ctx.clear()
+        self.cur_sample = 0
+
+    def render_one_pass(self, n_passes=1):
+        """renderOnePass; n_passes > 1 renders consecutive passes in one launch."""
+        self.free_counter += 1
+        self.ctx.render(_lib.INTEGRATOR_NAIVE, self.cur_sample, n_passes, self.free_counter, self.max_depth,
+                        self.russian_roulette, self.sample_light, self.light_env_uniform, self.light_portion)
+        self.free_counter += n_passes - 1
+        self.cur_sample += n_passes
+
+    def get_frame(self):
+        return self.ctx.read_frame()
+
+    def result_scale(self):
+        # 1 / (mCurSample + 1) as the reference displays it (Integrator.h:79)
+        return 1.0 / (self.cur_sample + 1)
+
+
+class LightPathIntegrator:
+    """Light tracer (src/integrator/LightPath.cpp:83-135); LightPathIntegParam defaults."""
+    WORKGROUP = 1536
+
+    def __init__(self, max_depth=4, russian_roulette=False, thread_blocks_one_pass=32):
+        self.max_depth = max_depth
+        self.russian_roulette = russian_roulette
+        self.blocks = thread_blocks_one_pass
+        self.cur_sample = 0
+        self.free_counter = 0
+        self.sample_per_pixel = 0.0
+
+    def init(self, ctx):
+        self.ctx = ctx
+        self.reset()
+
+    def reset(self):
+        self.ctx.clear()
+        self.cur_sample = 0
+        self.sample_per_pixel = self.blocks * self.WORKGROUP / (self.ctx.width * self.ctx.height)
+
+    def render_one_pass(self, n_passes=1):
+        self.free_counter += 1
+        self.ctx.render(_lib.INTEGRATOR_LIGHT, self.cur_sample, n_passes, self.free_counter, self.max_depth,
+                        self.russian_roulette, blocks_one_pass=self.blocks)
+        self.free_counter += n_passes - 1
+        self.sample_per_pixel += n_passes * self.blocks * self.WORKGROUP / (self.ctx.width * self.ctx.height)
+        self.cur_sample += n_passes
+
+    def get_frame(self):
+        return self.ctx.read_frame()
+
+    def result_scale(self):
+        return 1.0 / self.sample_per_pixel
+
+
+class TriplePathIntegrator:
+    """s=0 / s=1 / t=1 MIS (src/integrator/TriplePath.cpp:97-156); TriplePathIntegParam defaults."""
+
+    def __init__(self, max_depth=4, russian_roulette=False, lpt_blocks_one_pass=64, lpt_loops_per_pass=1):
+        self.max_depth = max_depth
+        self.russian_roulette = russian_roulette
+        self.blocks = lpt_blocks_one_pass
+        self.loops = lpt_loops_per_pass
+        self.cur_sample = 0
+        self.free_counter = 0
+        self.sample_per_pixel = 1.0
+
+    def init(self, ctx):
+        self.ctx = ctx
+        self.reset()
+
+    def reset(self):
+        self.ctx.clear()
+        self.cur_sample = 0
+        self.sample_per_pixel = 1.0
+
+    def render_one_pass(self, n_passes=1):
+        self.free_counter += 1
+        self.ctx.render(_lib.INTEGRATOR_TRIPLE, self.cur_sample, n_passes, self.free_counter, self.max_depth,
+                        self.russian_roulette, blocks_one_pass=self.blocks, loops_per_pass=self.loops)
+        self.free_counter += n_passes - 1
+        self.sample_per_pixel += n_passes
+        self.cur_sample += n_passes
+
+    def get_frame(self):
+        return self.ctx.read_frame()
+
+    def result_scale(self):
+        return 1.0 / self.sample_per_pixel

@@ -1,0 +1,546 @@
+"""Scene description, packing into the reference's buffer layout, and the synthetic
+workloads the renderer is measured on.
+
+Packing follows Scene::createGLContext (src/core/Scene.cpp:133-270): objects are
+flattened first (vertices, normals, texcoords, indices offset, one matTexIndex per
+triangle = offMaterial + (texIndex << 16 | matIndex)), then light meshes (no
+texcoords, no matTexIndex); the MTBVH is built over everything; light triangle
+power = radiance * area_i / sum(area) with a Rec.601-luminance alias table; the env
+map is stored as RGB16F with a two-level alias table built from FP32 data.
+
+Generated meshes are given in world space (the reference pre-transforms vertices by
+the model matrix before packing, so the packed buffers are the same either way).
+"""
+from dataclasses import dataclass, field
+import math
+
+import numpy as np
+
+from . import _lib
+
+MAT_LAMBERTIAN, MAT_PRINCIPLED, MAT_METAL_WORKFLOW, MAT_DIELECTRIC, MAT_THIN_DIELECTRIC = range(5)
+F32 = np.float32
+
+
+@dataclass
+class Material:
+    """The 64-byte material record (src/core/Material.h:32-53) with its defaults."""
+    base_color: tuple = (1.0, 1.0, 1.0)
+    roughness: float = 1.0
+    subsurface: float = 0.0
+    metallic: float = 0.0
+    specular: float = 1.0
+    specular_tint: float = 1.0
+    sheen: float = 0.0
+    sheen_tint: float = 1.0
+    clearcoat: float = 0.0
+    clearcoat_gloss: float = 0.0
+    ior: float = 1.5
+    type: int = MAT_LAMBERTIAN
+
+    def record(self):
+        r = np.zeros(16, np.float32)
+        r[0:3] = self.base_color
+        r[3] = self.roughness
+        r[4:8] = (self.subsurface, self.metallic, self.specular, self.specular_tint)
+        r[8:12] = (self.sheen, self.sheen_tint, self.clearcoat, self.clearcoat_gloss)
+        r[12] = self.ior
+        r.view(np.int32)[13] = self.type
+        return r
+
+
+@dataclass
+class Mesh:
+    positions: np.ndarray            # (n, 3) float32
+    normals: np.ndarray              # (n, 3) float32
+    indices: np.ndarray              # (m, 3) uint32
+    texcoords: np.ndarray = None     # (n, 2) float32 or None
+
+
+@dataclass
+class CameraDesc:
+    """Camera as the XML scene describes it (src/core/Scene.cpp:85-111)."""
+    pos: tuple = (0.0, 0.0, 0.0)
+    angle: tuple = (90.0, 0.0, 0.0)   # yaw, pitch in degrees; roll in radians
+    fov: float = 45.0
+    lens_radius: float = 0.0
+    focal_dist: float = 1.0
+
+
+@dataclass
+class SceneDesc:
+    objects: list = field(default_factory=list)   # [(Mesh, Material)]
+    lights: list = field(default_factory=list)    # [(Mesh, (r, g, b) power)]
+    env: np.ndarray = None                        # (h, w, 3) float32
+    env_rotation: float = 0.0
+    camera: CameraDesc = field(default_factory=CameraDesc)
+    width: int = 256
+    height: int = 256
+    sampler: int = 1                              # 1 = Sobol
+    noise_seed: int = 0
+
+
+# ---------------------------------------------------------------------------
+# Counter-based PRNG used by every generator (splitmix64); identical in any language.
+# ---------------------------------------------------------------------------
+_GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+
+
+def splitmix64(seed, n, offset=0):
+    with np.errstate(over="ignore"):
+        x = np.uint64(seed) + (np.arange(offset + 1, offset + n + 1, dtype=np.uint64) * _GOLDEN)
+        z = x
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def uniform01(seed, n, offset=0):
+    """float32 uniforms in [0, 1): top 24 bits of splitmix64 times 2^-24."""
+    return ((splitmix64(seed, n, offset) >> np.uint64(40)).astype(np.float64) * (1.0 / 16777216.0)).astype(np.float32)
+
+
+class Rng:
+    def __init__(self, seed):
+        self.seed = seed
+        self.off = 0
+
+    def uniform(self, n=None, lo=0.0, hi=1.0):
+        k = 1 if n is None else int(np.prod(n))
+        u = uniform01(self.seed, k, self.off).astype(np.float64)
+        self.off += k
+        v = lo + (hi - lo) * u
+        return float(v[0]) if n is None else v.reshape(n)
+
+    def normal(self, n):
+        k = int(np.prod(n))
+        u1 = np.maximum(self.uniform(k), 1e-12)
+        u2 = self.uniform(k)
+        return (np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)).reshape(n)
+
+
+def noise_texture(width, height, seed=0):
+    """Per-pixel RG32F seed texture (the reference's uNoiseTex, src/core/Sampler.cpp:66-80;
+    its MSVC default_random_engine stream is replaced by splitmix64)."""
+    return uniform01(seed ^ 0x6E6F697365, width * height * 2).reshape(height, width, 2)
+
+
+# ---------------------------------------------------------------------------
+# Camera (src/core/Camera.cpp:149-162) and the uniforms NaivePath.cpp:49-60 derives
+# ---------------------------------------------------------------------------
+def _f(x):
+    return np.float32(x)
+
+
+def _normalize(v):
+    v = np.asarray(v, np.float32)
+    return (v * _f(_f(1.0) / np.sqrt(_f(np.dot(v, v))))).astype(np.float32)
+
+
+def camera_uniforms(cam: CameraDesc, width, height):
+    rad = lambda d: _f(d) * _f(0.01745329251994329576923690768489)
+    ax, ay = rad(cam.angle[0]), rad(cam.angle[1])
+    x = _f(np.sin(ax)) * _f(np.cos(ay))
+    y = _f(np.cos(ax)) * _f(np.cos(ay))
+    z = _f(np.sin(ay))
+    front = _normalize(np.array([x, y, z], np.float32))
+    right = _normalize(np.cross(front, np.array([0, 0, 1], np.float32)).astype(np.float32))
+    roll = _f(cam.angle[2])
+    if roll != 0:
+        # glm::rotate(mat4(1), roll, front) applied to right (Camera.cpp:159-160)
+        c, s = _f(np.cos(roll)), _f(np.sin(roll))
+        a = _normalize(front)
+        t = (_f(1) - c) * a
+        Rm = np.array([[c + t[0] * a[0], t[0] * a[1] + s * a[2], t[0] * a[2] - s * a[1]],
+                       [t[1] * a[0] - s * a[2], c + t[1] * a[1], t[1] * a[2] + s * a[0]],
+                       [t[2] * a[0] + s * a[1], t[2] * a[1] - s * a[0], c + t[2] * a[2]]], np.float32)
+        right = _normalize((Rm[0] * right[0] + Rm[1] * right[1] + Rm[2] * right[2]).astype(np.float32))
+    up = _normalize(np.cross(right, front).astype(np.float32))
+    # inverse(mat3(right, up, front)), column-major
+    Mcols = np.stack([right, up, front], axis=1).astype(np.float64)
+    inv = np.linalg.inv(Mcols).astype(np.float32)
+    fov = min(max(cam.fov, 0.1), 90.0)
+    cam_struct = _lib.Camera()
+    cam_struct.front[:] = front.tolist()
+    cam_struct.right[:] = right.tolist()
+    cam_struct.up[:] = up.tolist()
+    cam_struct.pos[:] = [float(v) for v in np.asarray(cam.pos, np.float32)]
+    cam_struct.mat_inv[:] = inv.T.reshape(-1).tolist()   # column-major
+    cam_struct.tan_fov = float(_f(np.tan(rad(_f(fov) * _f(0.5)))))
+    cam_struct.aspect = float(_f(width) / _f(height))
+    cam_struct.lens_radius = float(cam.lens_radius)
+    cam_struct.focal_dist = float(cam.focal_dist)
+    cam_struct.film_width = width
+    cam_struct.film_height = height
+    return cam_struct
+
+
+# ---------------------------------------------------------------------------
+# Packing
+# ---------------------------------------------------------------------------
+class PackedScene:
+    """Buffers in the reference layout (SceneGLContext) + the scalars the shaders read."""
+
+    def as_struct(self):
+        sb = _lib.SceneBuffers()
+        P = _lib.ptr
+        import ctypes as C
+        sb.vertices = P(self.vertices, C.c_float)
+        sb.normals = P(self.normals, C.c_float)
+        sb.n_vertices = self.vertices.shape[0]
+        sb.texcoords = P(self.texcoords, C.c_float)
+        sb.n_texcoords = self.texcoords.shape[0]
+        sb.indices = P(self.indices, C.c_int32)
+        sb.n_triangles = self.indices.shape[0]
+        sb.bounds = P(self.bounds, C.c_float)
+        sb.hit_table = P(self.hit_table, C.c_int32)
+        sb.bvh_size = self.bounds.shape[0]
+        sb.mat_tex_indices = P(self.mat_tex_indices, C.c_int32)
+        sb.obj_prim_count = self.obj_prim_count
+        sb.materials = P(self.materials, C.c_float)
+        sb.n_materials = self.materials.shape[0]
+        sb.light_power = P(self.light_power, C.c_float)
+        sb.light_alias = P(self.light_alias, C.c_int32)
+        sb.light_prob = P(self.light_prob, C.c_float)
+        sb.n_light_triangles = self.n_light_triangles
+        sb.light_sum = self.light_sum
+        sb.env_rgb16f = P(self.env_half, C.c_uint16)
+        sb.env_width = self.env_width
+        sb.env_height = self.env_height
+        sb.env_alias = P(self.env_alias, C.c_int32)
+        sb.env_prob = P(self.env_prob, C.c_float)
+        sb.env_sum = self.env_sum
+        sb.env_rotation = self.env_rotation
+        sb.sampler = self.sampler
+        return sb
+
+
+def _tri_area2(va, vb, vc):
+    """glm::length(glm::cross(vc - va, vb - va)) in float32."""
+    e1 = (vc - va).astype(np.float32)
+    e2 = (vb - va).astype(np.float32)
+    cx = e1[..., 1] * e2[..., 2] - e1[..., 2] * e2[..., 1]
+    cy = e1[..., 2] * e2[..., 0] - e1[..., 0] * e2[..., 2]
+    cz = e1[..., 0] * e2[..., 1] - e1[..., 1] * e2[..., 0]
+    return np.sqrt((cx * cx + cy * cy) + cz * cz).astype(np.float32)
+
+
+def _lum601(c):
+    return (_f(c[0]) * _f(0.299) + _f(c[1]) * _f(0.587)) + _f(c[2]) * _f(0.114)
+
+
+def pack(desc: SceneDesc) -> PackedScene:
+    verts, norms, texs, inds, mti, mats = [], [], [], [], [], []
+    off_v, off_m, obj_prims = 0, 0, 0
+    for mesh, mat in desc.objects:
+        mats.append(mat.record())
+        n = mesh.positions.shape[0]
+        verts.append(mesh.positions.astype(np.float32))
+        norms.append(mesh.normals.astype(np.float32))
+        tc = mesh.texcoords if mesh.texcoords is not None else np.zeros((n, 2), np.float32)
+        texs.append(tc.astype(np.float32))
+        inds.append(mesh.indices.astype(np.int64) + off_v)
+        m = mesh.indices.shape[0]
+        mti.append(np.full(m, off_m + ((-1 << 16) | 0), np.int64))
+        off_v += n
+        obj_prims += m
+        off_m += 1
+    n_tex = off_v
+    light_power, pdf = [], []
+    light_sum = _f(0.0)
+    for mesh, power in desc.lights:
+        n = mesh.positions.shape[0]
+        verts.append(mesh.positions.astype(np.float32))
+        norms.append(mesh.normals.astype(np.float32))
+        inds.append(mesh.indices.astype(np.int64) + off_v)
+        p = mesh.positions.astype(np.float32)
+        ii = mesh.indices
+        areas = _tri_area2(p[ii[:, 0]], p[ii[:, 1]], p[ii[:, 2]])
+        sum_area = _f(0.0)
+        for a in areas:
+            sum_area = _f(sum_area + a)
+        pw = np.asarray(power, np.float32)
+        for a in areas:
+            tp = ((pw * a) / sum_area).astype(np.float32)
+            light_power.append(tp)
+            lum = _lum601(tp)
+            pdf.append(lum)
+            light_sum = _f(light_sum + lum)
+        off_v += n
+    if not mats:
+        mats.append(Material().record())
+    ps = PackedScene()
+    ps.vertices = np.ascontiguousarray(np.concatenate(verts), np.float32)
+    ps.normals = np.ascontiguousarray(np.concatenate(norms), np.float32)
+    ps.texcoords = np.ascontiguousarray(np.concatenate(texs) if texs else np.zeros((0, 2), np.float32), np.float32)
+    ps.indices = np.ascontiguousarray(np.concatenate(inds), np.int32)
+    ps.mat_tex_indices = np.ascontiguousarray(
+        np.concatenate(mti).astype(np.int64).astype(np.int32) if mti else np.zeros(0, np.int32))
+    ps.materials = np.ascontiguousarray(np.stack(mats), np.float32)
+    ps.obj_prim_count = obj_prims
+    ps.n_light_triangles = ps.indices.shape[0] - obj_prims
+    ps.bounds, ps.hit_table = _lib.build_mtbvh(ps.vertices, ps.indices.astype(np.uint32))
+    if light_power:
+        ps.light_power = np.ascontiguousarray(np.stack(light_power), np.float32)
+        ps.light_alias, ps.light_prob = _lib.alias_build(np.asarray(pdf, np.float32))
+    else:
+        ps.light_power = np.zeros((0, 3), np.float32)
+        ps.light_alias = np.zeros(0, np.int32)
+        ps.light_prob = np.zeros(0, np.float32)
+    ps.light_sum = float(light_sum)
+    env = desc.env if desc.env is not None else np.zeros((1, 1, 3), np.float32)
+    env = np.ascontiguousarray(env, np.float32)
+    ps.env_height, ps.env_width = env.shape[0], env.shape[1]
+    ps.env_alias, ps.env_prob, env_sum = _lib.env_alias_build(env)
+    ps.env_sum = float(int(env_sum)) if math.isfinite(env_sum) else 0.0   # EnvironmentMap::sumPdf() -> int
+    ps.env_half = np.ascontiguousarray(_lib.float_to_half(env))
+    ps.env_rotation = float(desc.env_rotation)
+    ps.sampler = int(desc.sampler)
+    ps.n_tex = n_tex
+    ps.camera = camera_uniforms(desc.camera, desc.width, desc.height)
+    ps.width, ps.height = desc.width, desc.height
+    ps.noise = np.ascontiguousarray(noise_texture(desc.width, desc.height, desc.noise_seed))
+    return ps
+
+
+# ---------------------------------------------------------------------------
+# Geometry helpers
+# ---------------------------------------------------------------------------
+def quad(p0, p1, p2, p3, normal):
+    pos = np.array([p0, p1, p2, p3], np.float32)
+    nrm = np.tile(np.asarray(normal, np.float32), (4, 1))
+    idx = np.array([[0, 1, 2], [0, 2, 3]], np.uint32)
+    uv = np.array([[0, 0], [1, 0], [1, 1], [0, 1]], np.float32)
+    return Mesh(pos, nrm, idx, uv)
+
+
+def merge(meshes):
+    pos, nrm, idx, uv, off = [], [], [], [], 0
+    for m in meshes:
+        pos.append(m.positions)
+        nrm.append(m.normals)
+        idx.append(m.indices + off)
+        uv.append(m.texcoords if m.texcoords is not None else np.zeros((len(m.positions), 2), np.float32))
+        off += len(m.positions)
+    return Mesh(np.concatenate(pos).astype(np.float32), np.concatenate(nrm).astype(np.float32),
+                np.concatenate(idx).astype(np.uint32), np.concatenate(uv).astype(np.float32))
+
+
+def box(center, size, angle_deg):
+    """Axis-aligned box rotated about the vertical (z) axis: 12 triangles, flat normals."""
+    cx, cy, z0 = center
+    sx, sy, sz = size
+    a = math.radians(angle_deg)
+    c, s = math.cos(a), math.sin(a)
+    rot = lambda x, y: (cx + c * x - s * y, cy + s * x + c * y)
+    hx, hy = sx / 2, sy / 2
+    corners = [rot(-hx, -hy), rot(hx, -hy), rot(hx, hy), rot(-hx, hy)]
+    faces = []
+    bot = [(x, y, z0) for x, y in corners]
+    top = [(x, y, z0 + sz) for x, y in corners]
+    faces.append(quad(top[0], top[1], top[2], top[3], (0, 0, 1)))
+    faces.append(quad(bot[3], bot[2], bot[1], bot[0], (0, 0, -1)))
+    for i in range(4):
+        j = (i + 1) % 4
+        ex, ey = corners[j][0] - corners[i][0], corners[j][1] - corners[i][1]
+        n = np.array([ey, -ex, 0.0])
+        n /= np.linalg.norm(n)
+        faces.append(quad(bot[i], bot[j], top[j], top[i], n))
+    return merge(faces)
+
+
+def icosphere(subdiv):
+    """Unit icosphere: 20 * 4^subdiv triangles, shared vertices."""
+    t = (1.0 + 5 ** 0.5) / 2.0
+    v = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+                  [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], np.float64)
+    f = np.array([[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2],
+                  [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5],
+                  [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1]], np.int64)
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    for _ in range(subdiv):
+        e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
+        e.sort(axis=1)
+        key = e[:, 0] * (1 << 32) + e[:, 1]
+        uniq, inv = np.unique(key, return_inverse=True)
+        a, b = uniq >> 32, uniq & 0xFFFFFFFF
+        mid = v[a] + v[b]
+        mid /= np.linalg.norm(mid, axis=1, keepdims=True)
+        base = len(v)
+        v = np.concatenate([v, mid])
+        m = inv.reshape(3, -1).T + base  # midpoints of edges 01, 12, 20
+        m01, m12, m20 = m[:, 0], m[:, 1], m[:, 2]
+        f = np.concatenate([np.stack([f[:, 0], m01, m20], 1), np.stack([f[:, 1], m12, m01], 1),
+                            np.stack([f[:, 2], m20, m12], 1), np.stack([m01, m12, m20], 1)])
+    return v, f
+
+
+def vertex_normals(pos, idx):
+    p = pos.astype(np.float64)
+    fn = np.cross(p[idx[:, 1]] - p[idx[:, 0]], p[idx[:, 2]] - p[idx[:, 0]])
+    n = np.zeros_like(p)
+    for k in range(3):
+        np.add.at(n, idx[:, k], fn)
+    n /= np.maximum(np.linalg.norm(n, axis=1, keepdims=True), 1e-30)
+    return n.astype(np.float32)
+
+
+def uv_sphere(center, radius, slices=100, stacks=51):
+    """UV sphere with 2*slices + 2*slices*(stacks-2) triangles (10,000 for 100 x 51)."""
+    verts = [(0.0, 0.0, 1.0)]
+    for i in range(1, stacks):
+        th = math.pi * i / stacks
+        for j in range(slices):
+            ph = 2 * math.pi * j / slices
+            verts.append((math.sin(th) * math.cos(ph), math.sin(th) * math.sin(ph), math.cos(th)))
+    verts.append((0.0, 0.0, -1.0))
+    v = np.array(verts)
+    idx = []
+    for j in range(slices):
+        idx.append((0, 1 + j, 1 + (j + 1) % slices))
+    for i in range(stacks - 2):
+        r0, r1 = 1 + i * slices, 1 + (i + 1) * slices
+        for j in range(slices):
+            a, b, c, d = r0 + j, r0 + (j + 1) % slices, r1 + j, r1 + (j + 1) % slices
+            idx.append((a, c, b))
+            idx.append((b, c, d))
+    last = len(v) - 1
+    r0 = 1 + (stacks - 2) * slices
+    for j in range(slices):
+        idx.append((r0 + j, last, r0 + (j + 1) % slices))
+    idx = np.array(idx, np.uint32)
+    pos = (np.asarray(center) + radius * v).astype(np.float32)
+    return Mesh(pos, v.astype(np.float32), idx, None)
+
+
+# ---------------------------------------------------------------------------
+# Workloads
+# ---------------------------------------------------------------------------
+WHITE = (0.73, 0.73, 0.73)
+RED = (0.65, 0.05, 0.05)
+GREEN = (0.12, 0.45, 0.15)
+
+
+def cornell_objects():
+    """Unit-cube Cornell box, z up, open towards -y: 5 walls (10 tris) + 2 boxes (24 tris)."""
+    white = Material(base_color=WHITE)
+    floor = quad((0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1))
+    ceil = quad((0, 0, 1), (0, 1, 1), (1, 1, 1), (1, 0, 1), (0, 0, -1))
+    back = quad((0, 1, 0), (1, 1, 0), (1, 1, 1), (0, 1, 1), (0, -1, 0))
+    left = quad((0, 0, 0), (0, 1, 0), (0, 1, 1), (0, 0, 1), (1, 0, 0))
+    right = quad((1, 0, 0), (1, 0, 1), (1, 1, 1), (1, 1, 0), (-1, 0, 0))
+    short = box((0.66, 0.34, 0.0), (0.3, 0.3, 0.3), -17.0)
+    tall = box((0.34, 0.64, 0.0), (0.3, 0.3, 0.6), 17.0)
+    return [(merge([floor, ceil, back]), white), (left, Material(base_color=RED)),
+            (right, Material(base_color=GREEN)), (merge([short, tall]), white)]
+
+
+def cornell_light():
+    return quad((0.4, 0.4, 0.999), (0.4, 0.6, 0.999), (0.6, 0.6, 0.999), (0.6, 0.4, 0.999), (0, 0, -1))
+
+
+def config1(width=256, height=256):
+    """Cornell box, 36 triangles, ceiling quad light of radiance (17, 12, 4), 40 deg fov."""
+    d = SceneDesc()
+    d.objects = cornell_objects()
+    d.lights = [(cornell_light(), (17.0, 12.0, 4.0))]
+    d.camera = CameraDesc(pos=(0.5, -1.5, 0.5), angle=(0.0, 0.0, 0.0), fov=40.0)
+    d.width, d.height = width, height
+    return d
+
+
+def ray_batch(n=1_000_000, seed=0, lo=0.0, hi=1.0):
+    """Rays with origins uniform in the box [lo, hi]^3 and directions uniform on S^2 (SoA)."""
+    u = uniform01(seed, 5 * n).astype(np.float64).reshape(5, n)
+    o = lo + (hi - lo) * u[0:3]
+    z = 1.0 - 2.0 * u[3]
+    phi = 2.0 * np.pi * u[4]
+    r = np.sqrt(np.maximum(0.0, 1.0 - z * z))
+    d = np.stack([r * np.cos(phi), r * np.sin(phi), z])
+    return np.ascontiguousarray(np.concatenate([o, d]).astype(np.float32))
+
+
+def sky_env(width=2048, height=1024, seed=7, sun_theta=40.0, sun_phi=60.0, peak=5e3, sun_radius_deg=0.5):
+    """Sky gradient + Gaussian sun lobe + lognormal(0, 0.5) texel noise, FP32 (h, w, 3)."""
+    v = (np.arange(height) + 0.5) / height
+    u = (np.arange(width) + 0.5) / width
+    theta = v * np.pi
+    phi = u * 2 * np.pi
+    T, P = np.meshgrid(theta, phi, indexing="ij")
+    up = np.cos(T)
+    zen = np.array([0.25, 0.45, 1.0])
+    hor = np.array([0.9, 0.9, 1.0])
+    gnd = np.array([0.25, 0.22, 0.18])
+    k = np.clip(up, 0, 1)[..., None]
+    sky = hor * (1 - k) + zen * k
+    img = np.where((up >= 0)[..., None], sky, gnd * 0.5)
+    d = np.stack([np.sin(T) * np.cos(P), np.sin(T) * np.sin(P), np.cos(T)], -1)
+    st, sp = math.radians(sun_theta), math.radians(sun_phi)
+    sd = np.array([math.sin(st) * math.cos(sp), math.sin(st) * math.sin(sp), math.cos(st)])
+    ang = np.arccos(np.clip(d @ sd, -1, 1))
+    sig = math.radians(sun_radius_deg)
+    img = img + (peak * np.exp(-0.5 * (ang / sig) ** 2))[..., None] * np.array([1.0, 0.95, 0.85])
+    noise = np.exp(0.5 * Rng(seed).normal((height, width)))
+    img = img * noise[..., None]
+    return np.ascontiguousarray(img.astype(np.float32))
+
+
+def config2(width=1920, height=1080, n_spheres=40, subdiv=5, env_size=(2048, 1024), seed=1):
+    """40 displaced icospheres (subdivision 5) on a 200-triangle ground, per-object Disney
+    materials, 2048 x 1024 sky environment, no area lights."""
+    rng = Rng(seed)
+    d = SceneDesc()
+    base_v, base_f = icosphere(subdiv)
+    placed = []
+    while len(placed) < n_spheres:
+        r = rng.uniform(None, 0.4, 1.2)
+        x, y = rng.uniform(None, -9.0, 9.0), rng.uniform(None, -9.0, 9.0)
+        if all((x - px) ** 2 + (y - py) ** 2 > (r + pr + 0.1) ** 2 for px, py, pr in placed):
+            placed.append((x, y, r))
+    for (x, y, r) in placed:
+        amp = rng.uniform(None, 0.0, 0.05)
+        disp = 1.0 + amp * (2.0 * rng.uniform(len(base_v)) - 1.0)
+        pos = (np.array([x, y, r]) + r * base_v * disp[:, None]).astype(np.float32)
+        nrm = vertex_normals(pos, base_f)
+        mat = Material(base_color=tuple(rng.uniform(3).tolist()), roughness=rng.uniform(None, 0.05, 1.0),
+                       metallic=1.0 if rng.uniform(None) < 0.3 else 0.0, specular=0.5, type=MAT_PRINCIPLED)
+        d.objects.append((Mesh(pos, nrm, base_f.astype(np.uint32), None), mat))
+    g = []
+    n = 10
+    for i in range(n):
+        for j in range(n):
+            x0, x1 = -12 + 24 * i / n, -12 + 24 * (i + 1) / n
+            y0, y1 = -12 + 24 * j / n, -12 + 24 * (j + 1) / n
+            g.append(quad((x0, y0, 0), (x1, y0, 0), (x1, y1, 0), (x0, y1, 0), (0, 0, 1)))
+    d.objects.append((merge(g), Material(base_color=(0.5, 0.5, 0.5), roughness=0.8, specular=0.5,
+                                         type=MAT_PRINCIPLED)))
+    d.env = sky_env(env_size[0], env_size[1], seed=seed + 6)
+    d.camera = CameraDesc(pos=(0.0, -17.0, 6.0), angle=(0.0, -16.0, 0.0), fov=45.0)
+    d.width, d.height = width, height
+    return d
+
+
+def config3(width=1024, height=1024, seed=3):
+    """Cornell box + 4 small emissive triangles (area 1e-3, radiance 50) + a 10k-triangle
+    smooth dielectric sphere (IOR 1.5) for caustics."""
+    rng = Rng(seed)
+    d = config1(width, height)
+    sphere = uv_sphere((0.5, 0.22, 0.14), 0.14)
+    d.objects.append((sphere, Material(base_color=(1.0, 1.0, 1.0), roughness=0.0, ior=1.5, type=MAT_DIELECTRIC)))
+    side = math.sqrt(2e-3)  # right isosceles triangle of area 1e-3
+    spots = [((0.05, 0.3, 0.7), (1, 0, 0)), ((0.95, 0.6, 0.6), (-1, 0, 0)),
+             ((0.3, 0.95, 0.8), (0, -1, 0)), ((0.7, 0.95, 0.3), (0, -1, 0))]
+    for (c, n) in spots:
+        c = np.array(c) + (rng.uniform(3) - 0.5) * 0.02
+        n = np.array(n, np.float64)
+        t1 = np.cross(n, [0, 0, 1.0]) if abs(n[2]) < 0.9 else np.cross(n, [1.0, 0, 0])
+        t1 /= np.linalg.norm(t1)
+        t2 = np.cross(n, t1)
+        p0, p1, p2 = c, c + side * t1, c + side * t2
+        # wind so that the geometric normal matches n
+        if np.dot(np.cross(p1 - p0, p2 - p0), n) < 0:
+            p1, p2 = p2, p1
+        m = Mesh(np.array([p0, p1, p2], np.float32), np.tile(n.astype(np.float32), (3, 1)),
+                 np.array([[0, 1, 2]], np.uint32), None)
+        d.lights.append((m, (50.0, 50.0, 50.0)))
+    return d
