@@ -46,7 +46,8 @@ def test_closest_hit_c1_1m_rays_bit_exact(ctx, c1):
     assert np.array_equal(prim, rp), f"{np.count_nonzero(prim != rp)} prim mismatches"
     assert np.array_equal(t.view(np.uint32), rt.view(np.uint32)), "t bits differ"
     assert np.array_equal(b, rb), "barycentric bins differ"
-    assert (prim >= 0).mean() > 0.99  # origins inside the closed box: nearly every ray hits
+    # the box is open on the camera side (y = 0), so roughly 1/7 of the rays escape
+    assert 0.8 < (prim >= 0).mean() < 0.95
 
 
 def test_closest_hit_counters_match_oracle(ctx, c1):
