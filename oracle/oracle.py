@@ -78,6 +78,12 @@ def lib():
                                         C.c_int, _fp, _u64, C.c_int, C.c_int]
         L.orc_render_light.argtypes = [C.POINTER(OrcScene), C.POINTER(OrcParams), C.c_int64, C.c_int64,
                                        C.POINTER(C.c_double), _u64, C.c_int, C.c_int]
+        L.orc_fresnel.argtypes = [C.c_float, C.c_float]
+        L.orc_fresnel.restype = C.c_float
+        L.orc_ggx.argtypes = [C.c_float, C.c_float]
+        L.orc_ggx.restype = C.c_float
+        L.orc_concentric.argtypes = [C.c_float, C.c_float, _fp]
+        L.orc_camera_roundtrip.argtypes = [C.POINTER(OrcScene), C.c_float, C.c_float, _fp]
         L.orc_bsdf_eval.argtypes = [_fp, _fp, _fp, _fp, C.c_int, _fp]
         L.orc_bsdf_sample.argtypes = [_fp, _fp, _fp, C.c_int, _fp, C.c_uint32, _fp]
         _lib = L
@@ -237,3 +243,24 @@ def bsdf_sample(mat_record, wo, n, u, mode=0, seed=1):
     lib().orc_bsdf_sample(_p(m, C.c_float), _p(a[0], C.c_float), _p(a[1], C.c_float), mode, _p(a[2], C.c_float),
                           seed, _p(out, C.c_float))
     return out[:3], float(out[3]), out[4:7], int(out[7:8].view(np.uint32)[0])
+
+
+def fresnel(cos_ti, eta):
+    return float(lib().orc_fresnel(cos_ti, eta))
+
+
+def camera_roundtrip(ps, u, v):
+    out = np.zeros(2, np.float32)
+    s = scene_struct(ps)
+    lib().orc_camera_roundtrip(C.byref(s), u, v, _p(out, C.c_float))
+    return out
+
+
+def ggx(cos_theta, alpha):
+    return float(lib().orc_ggx(cos_theta, alpha))
+
+
+def concentric(u, v):
+    out = np.zeros(2, np.float32)
+    lib().orc_concentric(u, v, _p(out, C.c_float))
+    return out

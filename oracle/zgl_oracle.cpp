@@ -1927,4 +1927,28 @@ int orc_bsdf_sample(const float* mat, const float* wo, const float* n, int mode,
     return 0;
 }
 
+float orc_fresnel(float cos_ti, float eta) { return Ctx<float>::fresnel(cos_ti, eta); }
+float orc_ggx(float cos_theta, float alpha) { return Ctx<float>::ggx(cos_theta, alpha); }
+float orc_concentric(float u, float v, float* out)
+{
+    V2<float> r = concentric(V2<float>(u, v));
+    out[0] = r.x;
+    out[1] = r.y;
+    return 0.0f;
+}
+
+// Camera round trip: raster position of the ray generated for (uv, u) (camera.glsl:62-91).
+int orc_camera_roundtrip(const orc_scene* s, float u, float v, float* uv_out)
+{
+    orc_params p = {};
+    Cnt cn;
+    Ctx<float> c(*s, p, cn, 0);
+    V3<float> o, d;
+    c.cameraRay(V2<float>(u, v), 0.0f, 0.0f, 0.0f, 0.0f, o, d);
+    V2<float> r = c.rasterPos(o, d);
+    uv_out[0] = r.x;
+    uv_out[1] = r.y;
+    return 0;
+}
+
 }  // extern "C"
