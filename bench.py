@@ -92,6 +92,7 @@ def main():
     ap.add_argument("--workload", default="c2", choices=["c1", "c2"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-count", action="store_true")
+    ap.add_argument("--threshold", type=int, default=0, help="persistent schedule regeneration threshold (0 = default)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES (PMC calibration) and exit")
     args = ap.parse_args()
@@ -122,7 +123,7 @@ def main():
     W, H, spp, depth = ps.width, ps.height, cfg["spp"], cfg["depth"]
 
     def step():
-        ctx.render(_lib.INTEGRATOR_NAIVE, 0, spp, 1, depth)
+        ctx.render(_lib.INTEGRATOR_NAIVE, 0, spp, 1, depth, regen_threshold=args.threshold)
 
     for i in range(args.warmup):
         step()
@@ -148,7 +149,7 @@ def main():
     roofline = None
     if not args.no_count:
         ctx.reset_counters()
-        ctx.render(_lib.INTEGRATOR_NAIVE, 0, spp, 1, depth, count_bytes=True)
+        ctx.render(_lib.INTEGRATOR_NAIVE, 0, spp, 1, depth, count_bytes=True, regen_threshold=args.threshold)
         ctx.synchronize()
         cnt = ctx.counters()
         algo_bytes = float(cnt[0])
@@ -167,7 +168,7 @@ def main():
                     "rows_per_traversal": round(float(cnt[1]) / max(float(cnt[3]), 1.0), 2),
                     "leaf_tests_per_traversal": round(float(cnt[2]) / max(float(cnt[3]), 1.0), 2),
                     "traversals_per_sample": round(float(cnt[3]) / samples, 3),
-                    "kernel": "k_naive<false> (zillumgl_amd/csrc/zgl_kernels.hip)"}
+                    "kernel": "k_naive_persistent<false> (zillumgl_amd/csrc/zgl_persistent.hip)"}
     ctx.close()
 
     cpu = None

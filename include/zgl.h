@@ -153,6 +153,8 @@ typedef struct zgl_pass_params {
     int32_t blocks_one_pass;      /* light: threadBlocksOnePass; triple: LPTBlocksOnePass */
     int32_t loops_per_pass;       /* triple: LPTLoopsPerPass */
     int32_t count_bytes;          /* accumulate algorithmic-byte counters (slower) */
+    int32_t regen_threshold;      /* naive: lanes of a wave64 still walking the MTBVH when the
+                                     wave stops to shade and reissue rays (1..63; 0 = default) */
 } zgl_pass_params;
 
 zgl_status zgl_render(zgl_context* ctx, const zgl_pass_params* params);

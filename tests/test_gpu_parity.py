@@ -89,6 +89,7 @@ def test_naive_c1_matches_oracle(ctx, c1):
 
 def test_naive_c1_pass_by_pass_equals_batched(ctx, c1):
     """16 launches of one pass give the same sums as one launch of 16 passes."""
+    # (the persistent kernel forms each pixel's sum in pass order within a launch)
     ctx.upload(c1)
     a = render.NaivePathIntegrator(max_depth=5)
     a.init(ctx)
@@ -168,3 +169,55 @@ def test_triple_matches_oracle(ctx, c3_small):
     err = _rel_err(gpu, ref, floor)
     print(f"[gpu] triple: max rel err {err[sig].max():.3g}", flush=True)
     assert err[sig].max() <= 1e-4
+
+
+@pytest.mark.parametrize("threshold", [1, 16, 40, 63])
+def test_persistent_thresholds_match_oracle(ctx, c2_small, threshold):
+    """Whatever the regeneration threshold, each pixel's sum is formed in pass order:
+    the frame is the oracle's bit for bit (passes 3..7, free counter 4)."""
+    print(f"[gpu] persistent threshold {threshold}", flush=True)
+    ctx.upload(c2_small)
+    ctx.clear()
+    ctx.render(_lib.INTEGRATOR_NAIVE, 3, 5, 4, 8, regen_threshold=threshold)
+    gpu = ctx.read_frame()[..., :3]
+    ref, _ = orc.render_pixels(c2_small, orc.params(0, 3, 5, 4, 8))
+    assert np.array_equal(gpu.view(np.uint32), ref.view(np.uint32))
+
+
+@pytest.mark.parametrize("kw", [dict(max_depth=5, russian_roulette=True), dict(max_depth=0),
+                                dict(max_depth=3, sample_light=False), dict(max_depth=4, light_env_uniform=True,
+                                                                            light_portion=0.3)])
+def test_naive_options_match_oracle(ctx, c1, kw):
+    ctx.upload(c1)
+    ctx.clear()
+    ctx.render(_lib.INTEGRATOR_NAIVE, 0, 4, 1, **kw)
+    gpu = ctx.read_frame()[..., :3]
+    ref, _ = orc.render_pixels(c1, orc.params(0, 0, 4, 1, **kw))
+    assert np.array_equal(gpu.view(np.uint32), ref.view(np.uint32)), kw
+
+
+def test_light_and_triple_with_rr_match_oracle(ctx):
+    ps = scene.pack(scene.config3(64, 64))
+    ctx.upload(ps)
+    ctx.clear()
+    ctx.render(_lib.INTEGRATOR_LIGHT, 0, 1, 1, 6, russian_roulette=True, blocks_one_pass=4)
+    g = ctx.read_film()
+    r, _ = orc.render_light(ps, orc.params(1, 0, 1, 1, 6, russian_roulette=True, blocks_one_pass=4))
+    fl = 1e-3 * r.mean()
+    m = r > fl
+    assert (np.abs(g - r) / np.maximum(r, fl))[m].max() <= 1e-4
+    ctx.clear()
+    ctx.render(_lib.INTEGRATOR_TRIPLE, 0, 1, 1, 6, russian_roulette=True, blocks_one_pass=2)
+    g = ctx.read_film()
+    pt, _ = orc.render_pixels(ps, orc.params(2, 0, 1, 1, 6, russian_roulette=True, blocks_one_pass=2))
+    lp, _ = orc.render_light(ps, orc.params(2, 0, 1, 1, 6, russian_roulette=True, blocks_one_pass=2))
+    r = pt + lp
+    fl = 1e-3 * r.mean()
+    m = r > fl
+    assert (np.abs(g - r) / np.maximum(r, fl))[m].max() <= 1e-4
+
+
+def test_render_rejects_bad_threshold(ctx, c1):
+    ctx.upload(c1)
+    with pytest.raises(_lib.ZglError):
+        ctx.render(_lib.INTEGRATOR_NAIVE, 0, 1, 1, 4, regen_threshold=64)

@@ -55,6 +55,7 @@ class PassParams(C.Structure):
         ("free_counter", C.c_int32), ("max_depth", C.c_int32), ("russian_roulette", C.c_int32),
         ("sample_light", C.c_int32), ("light_env_uniform", C.c_int32), ("light_portion", C.c_float),
         ("blocks_one_pass", C.c_int32), ("loops_per_pass", C.c_int32), ("count_bytes", C.c_int32),
+        ("regen_threshold", C.c_int32),
     ]
 
 

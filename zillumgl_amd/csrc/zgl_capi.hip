@@ -47,7 +47,9 @@ struct zgl_context {
     std::string err;
     DevBuf rows, tris, indices, vertices, normals, texcoords, matTex, materials;
     DevBuf lightPower, lightAlias, lightProb, env, envAlias, envProb, noise, sobolM, sobolTab;
-    DevBuf frame, film, counters, scratchRays, scratchOut;
+    DevBuf frame, film, counters, scratchRays, scratchOut, workCounter;
+    int numCUs = 0;
+    int naiveBlocksPerCU = 0;
     DevScene scene = {};
     bool hasScene = false, hasCamera = false, hasNoise = false;
     int filmW = 0, filmH = 0;
@@ -73,7 +75,7 @@ struct zgl_context {
     {
         DevBuf* all[] = {&rows, &tris, &indices, &vertices, &normals, &texcoords, &matTex, &materials,
                          &lightPower, &lightAlias, &lightProb, &env, &envAlias, &envProb, &noise, &sobolM,
-                         &sobolTab, &frame, &film, &counters, &scratchRays, &scratchOut};
+                         &sobolTab, &frame, &film, &counters, &scratchRays, &scratchOut, &workCounter};
         for (DevBuf* b : all) b->release();
     }
 };
@@ -115,6 +117,9 @@ template <class F> zgl_status guardedHost(F&& f)
 }
 
 }  // namespace
+
+// Lanes still walking when a wave of the persistent schedule stops to shade.
+constexpr int kDefaultRegenThreshold = 16;
 
 extern "C" {
 
@@ -189,6 +194,8 @@ zgl_status zgl_create(int32_t device, zgl_context** out)
         ctx->stream = ctx->own;
         ctx->upload(ctx->sobolM, sobolMatrices(), 256 * 32 * sizeof(uint32_t), "sobol matrices");
         ctx->ensure(ctx->counters, NumCounters * sizeof(unsigned long long), "counters");
+        ctx->ensure(ctx->workCounter, 256, "work counter");
+        hipCheck(hipDeviceGetAttribute(&ctx->numCUs, hipDeviceAttributeMultiprocessorCount, device), "attribute");
         hipCheck(hipMemsetAsync(ctx->counters.p, 0, NumCounters * sizeof(unsigned long long), ctx->stream),
                  "counters");
         hipCheck(hipStreamSynchronize(ctx->stream), "sync");
@@ -480,6 +487,7 @@ zgl_status zgl_render(zgl_context* ctx, const zgl_pass_params* p)
         if ((p->integrator == ZGL_INTEGRATOR_LIGHT || p->integrator == ZGL_INTEGRATOR_TRIPLE) &&
             ctx->scene.nLightTriangles < 1)
             invalid("light and triple integrators need light triangles");
+        if (p->regen_threshold < 0 || p->regen_threshold > 63) invalid("regen_threshold must be in [0, 63]");
         if (p->integrator != ZGL_INTEGRATOR_NAIVE && (p->blocks_one_pass < 1 || p->blocks_one_pass > 65536))
             invalid("blocks_one_pass");
         if (p->integrator == ZGL_INTEGRATOR_TRIPLE && (p->loops_per_pass < 1 || p->loops_per_pass > 1024))
@@ -510,7 +518,11 @@ zgl_status zgl_render(zgl_context* ctx, const zgl_pass_params* p)
         float4* frame = (float4*)ctx->frame.p;
         float* film = (float*)ctx->film.p;
         if (p->integrator == ZGL_INTEGRATOR_NAIVE) {
-            hipCheck(launchNaive(s, pp, frame, cnt, ctx->stream), "naive pass");
+            if (!ctx->naiveBlocksPerCU) ctx->naiveBlocksPerCU = naivePersistentBlocksPerCU();
+            const int threshold = p->regen_threshold > 0 ? p->regen_threshold : kDefaultRegenThreshold;
+            hipCheck(launchNaivePersistent(s, pp, frame, (int*)ctx->workCounter.p, threshold,
+                                           ctx->numCUs * ctx->naiveBlocksPerCU, cnt, ctx->stream),
+                     "naive pass");
         } else if (p->integrator == ZGL_INTEGRATOR_LIGHT) {
             s.sampler = 0;  // LightPath.cpp:49 forces the independent sampler
             hipCheck(launchLight(s, pp, film, cnt, ctx->stream), "light pass");

@@ -238,6 +238,54 @@ ZGL_D int traverse(const DevScene& s, const RayPre& p, float& dist, Counters& cn
     return closest;
 }
 
+// Resumable traversal: the state of one bvhHit / bvhTest walk, advanced one row per
+// call.  Visiting the rows in the same order with the same tests as traverse() gives
+// the same result bit for bit; the persistent kernels interleave many such walks.
+struct Walk {
+    RayPre p;
+    float dist;
+    int k;
+    int closest;
+    bool any;
+};
+template <bool C> ZGL_D void walkStart(Walk& w, const Ray& r, bool any, float tmax, Counters& cn)
+{
+    if (C) cn.traversals++;
+    w.p = prepareRay(r);
+    w.dist = tmax;
+    w.k = 0;
+    w.closest = -1;
+    w.any = any;
+}
+// One row; returns true when the walk has finished.
+template <bool C> ZGL_D bool walkStep(const DevScene& s, Walk& w, Counters& cn)
+{
+    const float4* rows = s.rows + ((size_t)w.p.table * (size_t)s.bvhSize + (size_t)w.k) * 2;
+    const float4 lo = rows[0];
+    const float4 hi = rows[1];
+    if (C) { cn.rows++; cn.bytes += 36; }
+    float boxDist;
+    const bool bHit = boxHit(v3(lo.x, lo.y, lo.z), v3(lo.w, hi.x, hi.y), w.p, boxDist);
+    if (!bHit || boxDist > w.dist) {
+        w.k = __float_as_int(hi.w);
+    } else {
+        const int prim = __float_as_int(hi.z);
+        if (prim >= 0) {
+            vec3 a, b, c;
+            loadTri(s, prim, a, b, c);
+            if (C) { cn.tests++; cn.bytes += 48; }
+            float t;
+            if (intersectTriangle(a, b, c, w.p.o, w.p.d, t) && t < w.dist) {
+                w.closest = prim;
+                if (w.any) return true;
+                w.dist = t;
+            }
+        }
+        w.k++;
+    }
+    return w.k == s.bvhSize;
+}
+
 template <bool C> ZGL_D int bvhHit(const DevScene& s, const Ray& r, float& dist, Counters& cn)
 {
     dist = 1e8f;

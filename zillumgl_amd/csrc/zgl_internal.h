@@ -36,8 +36,9 @@ hipError_t launchTraceClosest(const DevScene& s, const float* rays, int64_t n, i
                               uint32_t* bin, unsigned long long* counters, hipStream_t st);
 hipError_t launchTraceAny(const DevScene& s, const float* rays, const float* tmax, int64_t n,
                           uint8_t* occluded, hipStream_t st);
-hipError_t launchNaive(const DevScene& s, const PassParams& p, float4* frame, unsigned long long* counters,
-                       hipStream_t st);
+hipError_t launchNaivePersistent(const DevScene& s, const PassParams& p, float4* frame, int* pixelCounter,
+                                 int threshold, int blocks, unsigned long long* counters, hipStream_t st);
+int naivePersistentBlocksPerCU();
 hipError_t launchLight(const DevScene& s, const PassParams& p, float* film, unsigned long long* counters,
                        hipStream_t st);
 hipError_t launchTriplePT(const DevScene& s, const PassParams& p, float* film, unsigned long long* counters,

@@ -2,7 +2,7 @@
 //
 // Kernels mirror the reference's compute shaders:
 //   k_trace_closest   bvhHit over a batch of rays (intersection.glsl:395-427)
-//   k_naive           path_integ_naive.glsl:35-174 (one path per pixel per pass)
+//   (the naive path tracer's persistent kernel is in zgl_persistent.hip)
 //   k_light           light_path_integ.glsl:34-159
 //   k_triple_pt       triple_path_pass_pt.glsl:44-224
 //   k_triple_lpt      triple_path_pass_lpt.glsl:36-193
@@ -125,94 +125,6 @@ __global__ __launch_bounds__(kBlock) void k_trace_any(DevScene s, const float* _
     occ[i] = bvhTest<false>(s, r, tmax[i], cn) ? 1 : 0;
 }
 
-// ---------------------------------------------------------------------------
-// Naive MIS path tracer (path_integ_naive.glsl:35-143)
-// ---------------------------------------------------------------------------
-template <bool C>
-__device__ vec3 naiveTrace(const DevScene& s, const PassParams& prm, Ray ray, Rng& rng, Counters& cn)
-{
-    const bool uniform = prm.lightEnvUniform != 0;
-    const float portion = prm.lightPortion;
-    float primDist;
-    int id = bvhHit<C>(s, ray, primDist, cn);
-    vec3 pos = rayPoint(ray, primDist);
-    if (id == -1) return envLe<C>(s, ray.dir, cn);
-    if (id - s.objPrimCount >= 0) return lightLe<C>(s, id - s.objPrimCount, pos, -ray.dir, cn);
-
-    vec3 wo = -ray.dir;
-    vec3 result = v3(0.0f);
-    vec3 throughput = v3(1.0f);
-
-    for (int bounce = 1; bounce <= prm.maxDepth; bounce++) {
-        SurfaceInfo surf = triangleSurfaceInfo<C>(s, id, pos, cn);
-        int matTexId = s.matTexIndices[id];
-        cnt<C>(cn, 4);
-        int matId = matTexId & 0x0000ffff;
-        uint32_t matType = loadMaterialType<C>(s, matId, cn);
-        if (matType != MatDielectric && matType != MatThinDielectric) {
-            if (dot(surf.ns, wo) < 0.0f) flipNormals(surf);
-        }
-        BSDFParam mp = loadMaterial<C>(s, matType, matId, cn);
-
-        if (prm.sampleLight) {
-            float ud = sample1D<C>(s, rng, cn);
-            vec4 us = sample4D<C>(s, rng, cn);
-            LightLiSample ls = sampleLightAndEnv<C>(s, pos, ud, us, uniform, portion, cn);
-            if (ls.pdf > 0.0f) {
-                vec3 f = materialBSDF(matType, mp, wo, ls.wi, surf.ns, ModeRadiance);
-                float fp = materialPdf(matType, mp, wo, ls.wi, surf.ns);
-                float weight = ls.pdf * ls.pdf / (ls.pdf * ls.pdf + fp * fp);
-                result += f * throughput * satDot(surf.ns, ls.wi) * ls.coef * weight;
-            }
-        }
-
-        vec3 u3 = sample3D<C>(s, rng, cn);
-        BSDFSample bs = materialSample(matType, mp, surf.ns, wo, ModeRadiance, u3, rng);
-        vec3 wi = bs.wi;
-        float bsdfPdf = bs.pdf;
-        bool deltaBsdf = (bs.flag == FlagSpecRefl || bs.flag == FlagSpecTrans);
-        if (bsdfPdf < 1e-8f) break;
-        throughput *= bs.bsdf / bsdfPdf * (deltaBsdf ? 1.0f : absDot(surf.ns, wi));
-
-        ray = rayOffseted(pos, wi);
-        float dist;
-        int nextId = bvhHit<C>(s, ray, dist, cn);
-        int lightId = nextId - s.objPrimCount;
-        vec3 nextPos = rayPoint(ray, dist);
-
-        if (nextId == -1) {
-            vec3 radiance = envLe<C>(s, wi, cn);
-            float weight = 1.0f;
-            if (prm.sampleLight && !deltaBsdf) {
-                float envPdf = envPdfLi<C>(s, wi, cn) * pdfSelectEnv(s, uniform, portion);
-                weight = (envPdf <= 0.0f) ? 0.0f : bsdfPdf * bsdfPdf / (bsdfPdf * bsdfPdf + envPdf * envPdf);
-            }
-            result += radiance * throughput * weight;
-            break;
-        } else if (lightId >= 0) {
-            vec3 radiance = lightLe<C>(s, lightId, nextPos, -wi, cn);
-            float weight = 1.0f;
-            if (prm.sampleLight && !deltaBsdf) {
-                float lightPdf = lightPdfLi<C>(s, lightId, pos, nextPos, cn) *
-                                 pdfSelectLight<C>(s, lightId, uniform, portion, cn);
-                weight = (lightPdf <= 0.0f) ? 0.0f
-                                            : bsdfPdf * bsdfPdf / (bsdfPdf * bsdfPdf + lightPdf * lightPdf);
-            }
-            result += radiance * throughput * weight;
-            break;
-        }
-        if (prm.russianRoulette) {
-            float continueProb = fminG(maxComponent(bs.bsdf / bsdfPdf), 0.95f);
-            if (sample1D<C>(s, rng, cn) >= continueProb) break;
-            throughput /= continueProb;
-        }
-        id = nextId;
-        pos = nextPos;
-        wo = -wi;
-    }
-    return result;
-}
-
 // Per-pixel seeds (path_integ_naive.glsl:152-164): a double bilinear lookup into
 // the noise texture picks the scramble and the rand() seed.
 struct PixelSeed { uint32_t tx, ty; };
@@ -228,47 +140,6 @@ __device__ __forceinline__ int sampleOffsetOf(int spp)
     int off = spp * SobolSampleDim;
     if (off > SobolSampleNum * SobolSampleDim) off -= SobolSampleNum * SobolSampleDim;
     return off;
-}
-
-template <bool C>
-__global__ __launch_bounds__(kBlock) void k_naive(DevScene s, PassParams prm, float4* __restrict__ frame,
-                                                  unsigned long long* counters)
-{
-    const int tilesX = (s.filmW + 15) / 16, tilesY = (s.filmH + 15) / 16;
-    const int tile = xcdTile(blockIdx.x, gridDim.x);
-    Counters cn;
-    if (tile < tilesX * tilesY) {
-        int x, y;
-        tilePixel(tile, tilesX, x, y);
-        if (x < s.filmW && y < s.filmH) {
-            const vec2 scr = v2((float)x, (float)y) / v2((float)s.filmW, (float)s.filmH);
-            const PixelSeed ps = pixelSeed(s, scr);
-            const size_t pix = (size_t)y * s.filmW + x;
-            float4 mem = frame[pix];
-            bool stored = false;
-            for (int k = 0; k < prm.nPasses; k++) {
-                const int spp = prm.firstPass + k;
-                const uint32_t fc = (uint32_t)(prm.freeCounter + k);
-                Rng rng;
-                rng.randSeed = ps.tx * fc + ps.ty;
-                rng.sampleSeed = ps.tx * ps.ty;
-                rng.sampleOffset = sampleOffsetOf(spp);
-                rng.dim = 0;
-                if (C) cn.bytes += 64 + 32;  // two noise lookups, film read-modify-write
-                vec4 u = sample4D<C>(s, rng, cn);
-                Ray ray = cameraSampleRay(s, scr, u);
-                vec3 res = naiveTrace<C>(s, prm, ray, rng, cn);
-                vec3 last = (spp == 0) ? v3(0.0f) : v3(mem.x, mem.y, mem.z);
-                if (!hasNan(res)) {
-                    vec3 sum = last + res;
-                    mem = make_float4(sum.x, sum.y, sum.z, 1.0f);
-                    stored = true;
-                }
-            }
-            if (stored) frame[pix] = mem;
-        }
-    }
-    flushCounters<C>(cn, counters);
 }
 
 // ---------------------------------------------------------------------------
@@ -673,15 +544,6 @@ static inline unsigned tileGrid(const DevScene& s)
 {
     const int tiles = ((s.filmW + 15) / 16) * ((s.filmH + 15) / 16);
     return (unsigned)(((tiles + 7) / 8) * 8);  // a multiple of 8 so every XCD gets the same count
-}
-hipError_t launchNaive(const DevScene& s, const PassParams& p, float4* frame, unsigned long long* counters,
-                       hipStream_t st)
-{
-    if (counters)
-        hipLaunchKernelGGL(k_naive<true>, dim3(tileGrid(s)), dim3(kBlock), 0, st, s, p, frame, counters);
-    else
-        hipLaunchKernelGGL(k_naive<false>, dim3(tileGrid(s)), dim3(kBlock), 0, st, s, p, frame, counters);
-    return hipGetLastError();
 }
 hipError_t launchLight(const DevScene& s, const PassParams& p, float* film, unsigned long long* counters,
                        hipStream_t st)

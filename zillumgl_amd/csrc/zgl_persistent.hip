@@ -1,0 +1,365 @@
+// zgl_persistent.hip -- persistent, regenerating schedule of the naive MIS path tracer.
+//
+// Same estimator as path_integ_naive.glsl:35-174 (and k_naive), scheduled for wave64:
+//   * every lane owns one pixel at a time and renders that pixel's passes in order,
+//     so its running sum is formed exactly as one launch per pass would form it;
+//   * a lane's path is a state machine whose states end by issuing one ray (camera,
+//     shadow or extension).  Walks advance one MTBVH row per step for all lanes of the
+//     wave; when a ballot shows that no more than `threshold` lanes are still walking,
+//     the lanes whose walk finished run their shading transition and issue their next
+//     ray, so lanes with short walks do not wait for the longest walk of the wave;
+//   * lanes that finish all passes of their pixel take the next pixel from a global
+//     counter, one atomic per wave (ballot + prefix count over the lanes that need one).
+// NEE contributions are evaluated before the shadow walk with the throughput of that
+// bounce and added after it if the ray is unoccluded, which is the reference's order of
+// additions; the BSDF is sampled after the NEE samples are drawn, the reference's order
+// of random-number use.
+#include "zgl_internal.h"
+
+namespace zgl {
+
+namespace {
+
+enum : int {
+    kNewPixel = 0,
+    kNewSample,
+    kPrimary,   // camera ray walking / walked
+    kSurface,   // at a surface vertex, nothing issued yet
+    kShadow,    // NEE shadow ray walking / walked
+    kExtend,    // extension ray walking / walked
+    kFinish,    // sample complete
+    kDone,
+};
+
+struct PixelSeed2 { uint32_t tx, ty; };
+
+__device__ __forceinline__ PixelSeed2 seedOf(const DevScene& s, vec2 scr)
+{
+    vec2 nc = noiseLookup(s, scr);
+    nc = noiseLookup(s, nc);
+    vec2 tc = v2((float)s.filmW, (float)s.filmH) * nc;
+    return PixelSeed2{(uint32_t)tc.x, (uint32_t)tc.y};
+}
+
+// Pixel order for the dynamic fetch: 8x8 tiles in row-major tile order.
+__device__ __forceinline__ void pixelOfIndex(int idx, int W, int& x, int& y)
+{
+    const int tilesX = (W + 7) / 8;
+    const int tile = idx >> 6, in = idx & 63;
+    x = (tile % tilesX) * 8 + (in & 7);
+    y = (tile / tilesX) * 8 + (in >> 3);
+}
+
+}  // namespace
+
+__global__ void k_reset_counter(int* c) { if (threadIdx.x == 0 && blockIdx.x == 0) c[0] = 0; }
+
+template <bool C> __device__ void flushCountersP(const Counters& cn, unsigned long long* out)
+{
+    if (!C) return;
+    atomicAdd(out + 0, (unsigned long long)cn.bytes);
+    atomicAdd(out + 1, (unsigned long long)cn.rows);
+    atomicAdd(out + 2, (unsigned long long)cn.tests);
+    atomicAdd(out + 3, (unsigned long long)cn.traversals);
+}
+
+template <bool C>
+__global__ __launch_bounds__(256) void k_naive_persistent(DevScene s, PassParams prm, float4* __restrict__ frame,
+                                                          int* __restrict__ pixelCounter, int threshold,
+                                                          unsigned long long* counters)
+{
+    const int W = s.filmW, H = s.filmH;
+    const int tilesX = (W + 7) / 8, tilesY = (H + 7) / 8;
+    const int nIndex = tilesX * tilesY * 64;
+    const bool uniform = prm.lightEnvUniform != 0;
+    const float portion = prm.lightPortion;
+    const int lane = threadIdx.x & 63;
+    Counters cn;
+
+    int phase = kNewPixel;
+    int pix = -1, pass = 0;
+    float4 mem = make_float4(0.f, 0.f, 0.f, 0.f);
+    bool stored = false;
+    vec2 scr = v2(0.f, 0.f);
+    PixelSeed2 ps = {0u, 0u};
+    Rng rng = {0u, 0u, 0, 0};
+    vec3 result = v3(0.f), throughput = v3(1.f), wo = v3(0.f), pos = v3(0.f), wi = v3(0.f);
+    int bounce = 0;
+    float bsdfPdf = 0.f, rrProb = 0.f;
+    bool delta = false, terminate = false;
+    vec3 pend = v3(0.f);
+    bool pendValid = false;
+    uint32_t pendBytes = 0;
+    bool walking = false;
+    Walk wk;
+    wk.p.o = v3(0.f); wk.p.d = v3(0.f); wk.p.dInv = v3(0.f); wk.p.mode = 0;
+    wk.p.nx = wk.p.ny = wk.p.nz = false; wk.p.table = 0;
+    wk.dist = 0.f; wk.k = 0; wk.closest = -1; wk.any = false;
+
+    for (;;) {
+        // ------------------------------------------------------------ shading
+        if (!walking) {
+            // completion of the ray that was walked
+            if (phase == kPrimary) {
+                const int id = wk.closest;
+                const Ray ray = makeRay(wk.p.o, wk.p.d);
+                pos = rayPoint(ray, wk.dist);
+                if (id == -1) {
+                    result = envLe<C>(s, ray.dir, cn);
+                    phase = kFinish;
+                } else if (id - s.objPrimCount >= 0) {
+                    result = lightLe<C>(s, id - s.objPrimCount, pos, -ray.dir, cn);
+                    phase = kFinish;
+                } else {
+                    wo = -ray.dir;
+                    result = v3(0.f);
+                    throughput = v3(1.f);
+                    bounce = 1;
+                    phase = (prm.maxDepth >= 1) ? kSurface : kFinish;
+                }
+            } else if (phase == kShadow) {
+                if (wk.closest < 0) {
+                    if (C) cn.bytes += pendBytes;
+                    if (pendValid) result += pend;
+                }
+                if (terminate) phase = kFinish;
+                else {
+                    walkStart<C>(wk, rayOffseted(pos, wi), false, 1e8f, cn);
+                    walking = true;
+                    phase = kExtend;
+                }
+            } else if (phase == kExtend) {
+                const int nextId = wk.closest;
+                const int lightId = nextId - s.objPrimCount;
+                const vec3 nextPos = rayPoint(makeRay(wk.p.o, wk.p.d), wk.dist);
+                if (nextId == -1) {
+                    vec3 radiance = envLe<C>(s, wi, cn);
+                    float weight = 1.0f;
+                    if (prm.sampleLight && !delta) {
+                        float envPdf = envPdfLi<C>(s, wi, cn) * pdfSelectEnv(s, uniform, portion);
+                        weight = (envPdf <= 0.0f) ? 0.0f : bsdfPdf * bsdfPdf / (bsdfPdf * bsdfPdf + envPdf * envPdf);
+                    }
+                    result += radiance * throughput * weight;
+                    phase = kFinish;
+                } else if (lightId >= 0) {
+                    vec3 radiance = lightLe<C>(s, lightId, nextPos, -wi, cn);
+                    float weight = 1.0f;
+                    if (prm.sampleLight && !delta) {
+                        float lightPdf = lightPdfLi<C>(s, lightId, pos, nextPos, cn) *
+                                         pdfSelectLight<C>(s, lightId, uniform, portion, cn);
+                        weight = (lightPdf <= 0.0f) ? 0.0f
+                                                    : bsdfPdf * bsdfPdf / (bsdfPdf * bsdfPdf + lightPdf * lightPdf);
+                    }
+                    result += radiance * throughput * weight;
+                    phase = kFinish;
+                } else {
+                    phase = kSurface;
+                    if (prm.russianRoulette) {
+                        if (sample1D<C>(s, rng, cn) >= rrProb) phase = kFinish;
+                        else throughput /= rrProb;
+                    }
+                    if (phase == kSurface) {
+                        wk.closest = nextId;
+                        pos = nextPos;
+                        wo = -wi;
+                        bounce++;
+                        if (bounce > prm.maxDepth) phase = kFinish;
+                    }
+                }
+            }
+
+            // a surface vertex: NEE sample + BSDF sample, issue the next ray
+            if (phase == kSurface) {
+                const int id = wk.closest;
+                SurfaceInfo surf = triangleSurfaceInfo<C>(s, id, pos, cn);
+                const int matId = s.matTexIndices[id] & 0x0000ffff;
+                cnt<C>(cn, 4);
+                const uint32_t matType = loadMaterialType<C>(s, matId, cn);
+                if (matType != MatDielectric && matType != MatThinDielectric) {
+                    if (dot(surf.ns, wo) < 0.0f) flipNormals(surf);
+                }
+                const BSDFParam mp = loadMaterial<C>(s, matType, matId, cn);
+                bool shadow = false;
+                Ray shadowRay;
+                float shadowDist = 0.f;
+                pendValid = false;
+                pendBytes = 0;
+                if (prm.sampleLight) {
+                    const float ud = sample1D<C>(s, rng, cn);
+                    const vec4 us = sample4D<C>(s, rng, cn);
+                    const float pdfSampleLight = pdfSampleLightOf(s, uniform, portion);
+                    const bool pickLight = ud < pdfSampleLight;
+                    const float pdfSelect = pickLight ? pdfSampleLight : 1.0f - pdfSampleLight;
+                    Counters deferred;  // bytes the reference reads only after an unoccluded test
+                    LightLiSample ls = invalidLi();
+                    bool valid = false;
+                    if (pickLight) {
+                        const int lid = lightSampleOne<C>(s, v2(us.x, us.y), cn);
+                        const int triId = lid + s.objPrimCount;
+                        vec3 a, b, c;
+                        triangleVerts<C>(s, triId, a, b, c, cn);
+                        const vec3 y = sampleTriangleUniform(a, b, c, v2(us.z, us.w));
+                        const vec3 wl = normalize(y - pos);
+                        const vec3 norm = triangleSurfaceInfo<C>(s, triId, y, cn).ng;
+                        const float cosTheta = dot(norm, -wl);
+                        if (cosTheta >= 1e-6f) {
+                            shadow = true;
+                            shadowRay = rayOffseted(pos, wl);
+                            const float dist = distance(pos, y);
+                            float pdf = dist * dist / (triangleArea(a, b, c) * cosTheta);
+                            shadowDist = dist - 1e-4f - 1e-6f;
+                            if (!(pdf < 1e-8f)) {
+                                const vec3 weight = lightLe<C>(s, lid, y, -wl, deferred);
+                                const float pdfSample = lightPdfSampleOne<C>(s, lid, deferred);
+                                pdf *= pdfSample;
+                                ls = LightLiSample{wl, weight / pdf, pdf};
+                                valid = true;
+                            }
+                        }
+                    } else {
+                        const vec4 sp = envSampleWi<C>(s, us, cn);
+                        const vec3 we = v3(sp.x, sp.y, sp.z);
+                        shadow = true;
+                        shadowRay = rayOffseted(pos, we);
+                        shadowDist = 1e8f;
+                        if (sp.w != 0.0f) {
+                            ls = LightLiSample{we, envLe<C>(s, we, deferred) / sp.w, sp.w};
+                            valid = true;
+                        }
+                    }
+                    if (valid) {
+                        ls.coef /= pdfSelect;
+                        ls.pdf *= pdfSelect;
+                        if (ls.pdf > 0.0f) {
+                            vec3 f = materialBSDF(matType, mp, wo, ls.wi, surf.ns, ModeRadiance);
+                            float fp = materialPdf(matType, mp, wo, ls.wi, surf.ns);
+                            float weight = ls.pdf * ls.pdf / (ls.pdf * ls.pdf + fp * fp);
+                            pend = f * throughput * satDot(surf.ns, ls.wi) * ls.coef * weight;
+                            pendValid = true;
+                        }
+                    }
+                    if (C) pendBytes = deferred.bytes;
+                }
+                const vec3 u3 = sample3D<C>(s, rng, cn);
+                const BSDFSample bs = materialSample(matType, mp, surf.ns, wo, ModeRadiance, u3, rng);
+                wi = bs.wi;
+                bsdfPdf = bs.pdf;
+                delta = (bs.flag == FlagSpecRefl || bs.flag == FlagSpecTrans);
+                terminate = bsdfPdf < 1e-8f;
+                if (!terminate) {
+                    throughput *= bs.bsdf / bsdfPdf * (delta ? 1.0f : absDot(surf.ns, wi));
+                    rrProb = fminG(maxComponent(bs.bsdf / bsdfPdf), 0.95f);
+                }
+                if (shadow) {
+                    walkStart<C>(wk, shadowRay, true, shadowDist, cn);
+                    walking = true;
+                    phase = kShadow;
+                } else if (terminate) {
+                    phase = kFinish;
+                } else {
+                    walkStart<C>(wk, rayOffseted(pos, wi), false, 1e8f, cn);
+                    walking = true;
+                    phase = kExtend;
+                }
+            }
+
+            if (phase == kFinish) {
+                if (!hasNan(result)) {
+                    const int spp = prm.firstPass + pass;
+                    const vec3 last = (spp == 0) ? v3(0.0f) : v3(mem.x, mem.y, mem.z);
+                    const vec3 sum = last + result;
+                    mem = make_float4(sum.x, sum.y, sum.z, 1.0f);
+                    stored = true;
+                }
+                pass++;
+                phase = (pass < prm.nPasses) ? kNewSample : kNewPixel;
+            }
+        }
+
+        // pixels are handed out per wave: one atomic for all lanes that need one
+        const unsigned long long need = __ballot(!walking && phase == kNewPixel);
+        if (need) {
+            if (!walking && phase == kNewPixel && pix >= 0 && stored) frame[pix] = mem;
+            const int n = __popcll(need);
+            const int leader = __ffsll((long long)need) - 1;
+            int base = 0;
+            if (lane == leader) base = atomicAdd(pixelCounter, n);
+            base = __shfl(base, leader);
+            if (!walking && phase == kNewPixel) {
+                const unsigned long long below = need & ((1ull << lane) - 1ull);
+                const int idx = base + __popcll(below);
+                pix = -1;
+                stored = false;
+                if (idx >= nIndex) {
+                    phase = kDone;
+                } else {
+                    int x, y;
+                    pixelOfIndex(idx, W, x, y);
+                    if (x < W && y < H) {  // slots of partial edge tiles fetch again next round
+                        pix = y * W + x;
+                        scr = v2((float)x, (float)y) / v2((float)W, (float)H);
+                        ps = seedOf(s, scr);
+                        mem = frame[pix];
+                        pass = 0;
+                        phase = kNewSample;
+                    }
+                }
+            }
+        }
+
+        if (!walking && phase == kNewSample) {
+            const int spp = prm.firstPass + pass;
+            const uint32_t fc = (uint32_t)(prm.freeCounter + pass);
+            rng.randSeed = ps.tx * fc + ps.ty;
+            rng.sampleSeed = ps.tx * ps.ty;
+            int off = spp * SobolSampleDim;
+            if (off > SobolSampleNum * SobolSampleDim) off -= SobolSampleNum * SobolSampleDim;
+            rng.sampleOffset = off;
+            rng.dim = 0;
+            if (C) cn.bytes += 64 + 32;
+            const vec4 u = sample4D<C>(s, rng, cn);
+            walkStart<C>(wk, cameraSampleRay(s, scr, u), false, 1e8f, cn);
+            walking = true;
+            phase = kPrimary;
+        }
+
+        if (__ballot(phase != kDone) == 0ull) break;
+
+        // ------------------------------------------------------------ walking
+        for (;;) {
+            if (walking) {
+                if (walkStep<C>(s, wk, cn)) walking = false;
+            }
+            if (__popcll(__ballot(walking)) <= threshold) break;
+        }
+    }
+    flushCountersP<C>(cn, counters);
+}
+
+}  // namespace zgl
+
+namespace zgl {
+
+hipError_t launchNaivePersistent(const DevScene& s, const PassParams& p, float4* frame, int* pixelCounter,
+                                 int threshold, int blocks, unsigned long long* counters, hipStream_t st)
+{
+    if (threshold < 0) threshold = 0;
+    if (threshold > 63) threshold = 63;
+    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, pixelCounter);
+    if (counters)
+        hipLaunchKernelGGL(k_naive_persistent<true>, dim3(blocks), dim3(256), 0, st, s, p, frame, pixelCounter,
+                           threshold, counters);
+    else
+        hipLaunchKernelGGL(k_naive_persistent<false>, dim3(blocks), dim3(256), 0, st, s, p, frame, pixelCounter,
+                           threshold, counters);
+    return hipGetLastError();
+}
+
+int naivePersistentBlocksPerCU()
+{
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_naive_persistent<false>, 256, 0) != hipSuccess) return 1;
+    return n > 0 ? n : 1;
+}
+
+}  // namespace zgl

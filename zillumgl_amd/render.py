@@ -90,10 +90,10 @@ class Context:
 
     def render(self, integrator, first_pass, n_passes, free_counter, max_depth, russian_roulette=False,
                sample_light=True, light_env_uniform=False, light_portion=0.5, blocks_one_pass=32,
-               loops_per_pass=1, count_bytes=False):
+               loops_per_pass=1, count_bytes=False, regen_threshold=0):
         p = _lib.PassParams(integrator, first_pass, n_passes, free_counter, max_depth, int(russian_roulette),
                             int(sample_light), int(light_env_uniform), light_portion, blocks_one_pass,
-                            loops_per_pass, int(count_bytes))
+                            loops_per_pass, int(count_bytes), regen_threshold)
         self._check(_lib.lib().zgl_render(self.h, C.byref(p)))
 
     def clear(self):
