@@ -73,6 +73,7 @@ def lib():
         L.orc_hash.restype = C.c_uint32
         L.orc_fmath.argtypes = [C.c_int, _fp, _fp, _fp, C.c_int64]
         L.orc_fmath.restype = None
+        L.orc_box_hit.argtypes = [_fp, _fp, C.c_int64, C.POINTER(C.c_uint8), _fp]
         L.orc_trace_closest.argtypes = [C.POINTER(OrcScene), _fp, C.c_int64, _ip, _fp, _up, _u64, C.c_int]
         L.orc_render_pixels.argtypes = [C.POINTER(OrcScene), C.POINTER(OrcParams), C.c_int, C.c_int, C.c_int,
                                         C.c_int, _fp, _u64, C.c_int, C.c_int]
@@ -181,6 +182,19 @@ def scene_struct(ps):
     s.noise = _p(ps.noise, C.c_float)
     s.noise_width, s.noise_height = ps.noise.shape[1], ps.noise.shape[0]
     return s
+
+
+def box_hit(boxes, rays):
+    """The box test on n (box, ray) pairs: boxes (n, 6) {pMin, pMax}, rays (n, 6) {o, d}.
+    Returns (hit bool array, entry distance where hit else 0)."""
+    bx = np.ascontiguousarray(boxes, np.float32)
+    r = np.ascontiguousarray(rays, np.float32)
+    n = bx.shape[0]
+    hit = np.zeros(n, np.uint8)
+    tmin = np.zeros(n, np.float32)
+    lib().orc_box_hit(_p(bx, C.c_float), _p(r, C.c_float), n, hit.ctypes.data_as(C.POINTER(C.c_uint8)),
+                      _p(tmin, C.c_float))
+    return hit.astype(bool), tmin
 
 
 def trace_closest(ps, rays, nthreads=0):

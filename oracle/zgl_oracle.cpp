@@ -1801,6 +1801,24 @@ void orc_fmath(int op, const float* in, const float* in2, float* out, int64_t n)
 
 static int threadsOf(int n) { return n > 0 ? n : (int)std::max(1u, std::thread::hardware_concurrency()); }
 
+// The box test alone (intersection.glsl:226-329) on n (box, ray) pairs: boxes n x 6
+// {pMin, pMax}, rays n x 6 {o, d}; writes hit and, where hit, the entry distance.
+int orc_box_hit(const float* boxes, const float* rays, int64_t n, uint8_t* hit, float* tmin)
+{
+    orc_scene sc = {};
+    sc.bounds = boxes;
+    orc_params p = {};
+    Cnt cn;
+    Ctx<float> c(sc, p, cn, 0);
+    for (int64_t i = 0; i < n; i++) {
+        const float* r = rays + 6 * i;
+        float t = 0.0f;
+        hit[i] = c.boxHit((int)i, V3<float>(r[0], r[1], r[2]), V3<float>(r[3], r[4], r[5]), t) ? 1 : 0;
+        tmin[i] = hit[i] ? t : 0.0f;
+    }
+    return 0;
+}
+
 int orc_trace_closest(const orc_scene* s, const float* rays, int64_t n, int32_t* prim, float* t, uint32_t* bin,
                       uint64_t* counters, int nthreads)
 {

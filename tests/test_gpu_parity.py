@@ -50,6 +50,55 @@ def test_closest_hit_c1_1m_rays_bit_exact(ctx, c1):
     assert 0.8 < (prim >= 0).mean() < 0.95
 
 
+def _special_rays(n, seed):
+    """Rays inside the Cornell box with directions on or near the axes, with components
+    at or below 1e-6, signed zeros, and a few NaN/inf components: every case of the box
+    test, mixed with general rays in the same waves."""
+    rng = np.random.default_rng(seed)
+    o = rng.uniform(0.05, 0.95, (n, 3)).astype(np.float32)
+    o[:, 1] = rng.uniform(-1.0, 0.95, n).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    d = d.astype(np.float32)
+    kind = rng.integers(0, 6, n)
+    axis = rng.integers(0, 3, n)
+    sgn = np.where(rng.random(n) < 0.5, -1.0, 1.0).astype(np.float32)
+    idx = np.arange(n)
+    m = kind == 1                             # exact axes, other components +0 / -0
+    d[m] = 0.0
+    d[idx[m], axis[m]] = sgn[m]
+    d[idx[m], (axis[m] + 1) % 3] = np.where(rng.random(m.sum()) < 0.5, -0.0, 0.0).astype(np.float32)
+    m = kind == 2                             # |d_a| around 1 - 1e-6
+    d[idx[m], axis[m]] = sgn[m] * rng.choice(np.float32([1 - 5e-7, 1 - 1e-6, 1 - 2e-6]), m.sum())
+    m = kind == 3                             # one component at or below 1e-6
+    d[idx[m], axis[m]] = rng.choice(np.float32([0.0, -0.0, 4e-7, -8e-7, 1e-6]), m.sum())
+    m = (kind == 4) & (rng.random(n) < 0.02)  # a few NaN / inf components
+    d[idx[m], axis[m]] = rng.choice(np.float32([np.nan, np.inf]), m.sum())
+    return np.ascontiguousarray(np.concatenate([o.T, d.T]).astype(np.float32))
+
+
+def test_closest_hit_special_rays_bit_exact(ctx, c1):
+    print("[gpu] closest hit, C1, special directions", flush=True)
+    ctx.upload(c1)
+    rays = _special_rays(400_000, seed=3)
+    prim, t, b = ctx.trace_closest(rays)
+    rp, rt, rb, _ = orc.trace_closest(c1, rays)
+    assert np.array_equal(prim, rp), f"{np.count_nonzero(prim != rp)} prim mismatches"
+    assert np.array_equal(t.view(np.uint32), rt.view(np.uint32)), "t bits differ"
+    assert np.array_equal(b, rb)
+    assert 0.3 < (prim >= 0).mean() < 0.99
+
+
+def test_upload_rejects_nonfinite_bounds(ctx, c1):
+    import copy
+    bad = copy.copy(c1)
+    bad.bounds = c1.bounds.copy()
+    bad.bounds[3, 4] = np.inf
+    with pytest.raises(_lib.ZglError):
+        ctx.upload(bad)
+    ctx.upload(c1)
+
+
 def test_closest_hit_counters_match_oracle(ctx, c1):
     ctx.upload(c1)
     rays = scene.ray_batch(100_000, seed=5)
@@ -173,8 +222,9 @@ def test_triple_matches_oracle(ctx, c3_small):
 
 @pytest.mark.parametrize("threshold", [1, 16, 40, 63])
 def test_persistent_thresholds_match_oracle(ctx, c2_small, threshold):
-    """Whatever the regeneration threshold, each pixel's sum is formed in pass order:
-    the frame is the oracle's bit for bit (passes 3..7, free counter 4)."""
+    """Whatever the regeneration threshold, each walk visits the same rows and each
+    pixel's sum is formed in pass order: the frame is the oracle's bit for bit (passes
+    3..7, free counter 4)."""
     print(f"[gpu] persistent threshold {threshold}", flush=True)
     ctx.upload(c2_small)
     ctx.clear()
@@ -221,3 +271,5 @@ def test_render_rejects_bad_threshold(ctx, c1):
     ctx.upload(c1)
     with pytest.raises(_lib.ZglError):
         ctx.render(_lib.INTEGRATOR_NAIVE, 0, 1, 1, 4, regen_threshold=64)
+    with pytest.raises(_lib.ZglError):
+        ctx.render(_lib.INTEGRATOR_NAIVE, 0, 1, 1, 4, regen_threshold=-1)

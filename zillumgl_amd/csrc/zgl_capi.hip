@@ -244,7 +244,8 @@ zgl_status zgl_upload_scene(zgl_context* ctx, const zgl_scene_buffers* sb)
         if (!sb) invalid("null scene");
         const int64_t nV = sb->n_vertices, nT = sb->n_triangles;
         if (nV < 3 || nT < 1 || !sb->vertices || !sb->normals || !sb->indices) invalid("empty geometry");
-        if (nT > (1LL << 29) || nV > (1LL << 30)) invalid("scene too large");
+        // absolute MTBVH row indices (6 tables x (2 nT - 1) rows) must fit in int32
+        if (nT > (1LL << 27) || nV > (1LL << 30)) invalid("scene too large");
         if (sb->bvh_size != 2 * nT - 1 || !sb->bounds || !sb->hit_table) invalid("bvh_size must be 2*n_triangles-1");
         if (sb->obj_prim_count < 0 || sb->obj_prim_count > nT) invalid("obj_prim_count out of range");
         if (sb->n_light_triangles != nT - sb->obj_prim_count) invalid("n_light_triangles != n_triangles - obj_prim_count");
@@ -259,6 +260,9 @@ zgl_status zgl_upload_scene(zgl_context* ctx, const zgl_scene_buffers* sb)
         for (int64_t i = 0; i < sb->obj_prim_count; i++)
             if ((sb->mat_tex_indices[i] & 0xffff) >= sb->n_materials) invalid("material index out of range");
         const int64_t B = sb->bvh_size;
+        // Finite bounds: the box test of general rays relies on NaN-free slab distances.
+        for (int64_t i = 0; i < B * 6; i++)
+            if (!(std::fabs(sb->bounds[i]) <= 3.4028235e38f)) invalid("non-finite MTBVH bounds");
         // Every row must point forward so that every traversal terminates.
         for (int t = 0; t < 6; t++)
             for (int64_t k = 0; k < B; k++) {
@@ -282,7 +286,8 @@ zgl_status zgl_upload_scene(zgl_context* ctx, const zgl_scene_buffers* sb)
             if (a < 0 || a >= h) invalid("env marginal alias out of range");
         }
 
-        // MTBVH row records: {pMin.xyz, pMax.x} {pMax.yz, prim, miss}
+        // MTBVH row records: {pMin.xyz, pMax.x} {pMax.yz, prim, miss}, the miss link
+        // made absolute (table * B + miss) so that a walk needs no per-table base
         std::vector<float> rows((size_t)6 * B * 8);
         for (int t = 0; t < 6; t++)
             for (int64_t k = 0; k < B; k++) {
@@ -291,8 +296,9 @@ zgl_status zgl_upload_scene(zgl_context* ctx, const zgl_scene_buffers* sb)
                 float* o = rows.data() + ((size_t)t * B + k) * 8;
                 o[0] = bb[0]; o[1] = bb[1]; o[2] = bb[2]; o[3] = bb[3];
                 o[4] = bb[4]; o[5] = bb[5];
+                const int32_t miss = (int32_t)(t * B) + r[2];
                 std::memcpy(o + 6, &r[1], 4);
-                std::memcpy(o + 7, &r[2], 4);
+                std::memcpy(o + 7, &miss, 4);
             }
         std::vector<float> tris((size_t)nT * 12, 0.0f);
         for (int64_t i = 0; i < nT; i++) {

@@ -15,6 +15,7 @@
 // Floating-point evaluation order follows the GLSL source expression by expression.
 #pragma once
 
+#include "zgl_box.h"
 #include "zgl_math.h"
 
 namespace zgl {
@@ -89,86 +90,7 @@ ZGL_D Ray makeRay(vec3 o, vec3 d) { return Ray{o, d}; }
 ZGL_D vec3 rayPoint(const Ray& r, float t) { return r.ori + r.dir * t; }
 ZGL_D Ray rayOffseted(vec3 ori, vec3 dir) { return Ray{ori + dir * 1e-4f, dir}; }
 
-// Per-ray constants of boxHit (intersection.glsl:226-329).  1/d and the branch
-// selection depend only on the ray, so computing them once gives the same bits
-// as the per-box evaluation of the reference.
-struct RayPre {
-    vec3 o, d, dInv;
-    int mode;  // 0 general, 1/2/3: |d.x|/|d.y|/|d.z| > 1 - eps
-    bool nx, ny, nz;
-    int table;
-};
-ZGL_D RayPre prepareRay(const Ray& r)
-{
-    const float eps = 1e-6f;
-    RayPre p;
-    p.o = r.ori;
-    p.d = r.dir;
-    p.dInv = v3(1.0f / r.dir.x, 1.0f / r.dir.y, 1.0f / r.dir.z);
-    const float one = 1.0f - eps;
-    p.mode = (fabsG(r.dir.x) > one) ? 1 : (fabsG(r.dir.y) > one) ? 2 : (fabsG(r.dir.z) > one) ? 3 : 0;
-    p.nx = fabsG(r.dir.x) < eps;
-    p.ny = fabsG(r.dir.y) < eps;
-    p.nz = fabsG(r.dir.z) < eps;
-    p.table = cubemapFace(-r.dir);
-    return p;
-}
-
-ZGL_D bool boxHit(vec3 pMin, vec3 pMax, const RayPre& p, float& tMin)
-{
-    const vec3 o = p.o;
-    float tMax;
-    if (p.mode == 1) {
-        if (o.y > pMin.y && o.y < pMax.y && o.z > pMin.z && o.z < pMax.z) {
-            float ta = (pMin.x - o.x) * p.dInv.x, tb = (pMax.x - o.x) * p.dInv.x;
-            tMin = fminG(ta, tb); tMax = fmaxG(ta, tb);
-            return tMax >= 0.0f && tMax >= tMin;
-        }
-        return false;
-    }
-    if (p.mode == 2) {
-        if (o.x > pMin.x && o.x < pMax.x && o.z > pMin.z && o.z < pMax.z) {
-            float ta = (pMin.y - o.y) * p.dInv.y, tb = (pMax.y - o.y) * p.dInv.y;
-            tMin = fminG(ta, tb); tMax = fmaxG(ta, tb);
-            return tMax >= 0.0f && tMax >= tMin;
-        }
-        return false;
-    }
-    if (p.mode == 3) {
-        if (o.x > pMin.x && o.x < pMax.x && o.y > pMin.y && o.y < pMax.y) {
-            float ta = (pMin.z - o.z) * p.dInv.z, tb = (pMax.z - o.z) * p.dInv.z;
-            tMin = fminG(ta, tb); tMax = fmaxG(ta, tb);
-            return tMax >= 0.0f && tMax >= tMin;
-        }
-        return false;
-    }
-    vec3 vta = (pMin - o) * p.dInv;
-    vec3 vtb = (pMax - o) * p.dInv;
-    vec3 vtMin = vmin(vta, vtb);
-    vec3 vtMax = vmax(vta, vtb);
-    vec3 dt = vtMax - vtMin;
-    float tyz = vtMax.z - vtMin.y;
-    float tzx = vtMax.x - vtMin.z;
-    float txy = vtMax.y - vtMin.x;
-    if (p.nx && dt.y + dt.z > tyz) {
-        tMin = fmaxG(vtMin.y, vtMin.z); tMax = fminG(vtMax.y, vtMax.z);
-        return tMax >= 0.0f && tMax >= tMin;
-    }
-    if (p.ny && dt.z + dt.x > tzx) {
-        tMin = fmaxG(vtMin.z, vtMin.x); tMax = fminG(vtMax.z, vtMax.x);
-        return tMax >= 0.0f && tMax >= tMin;
-    }
-    if (p.nz && dt.x + dt.y > txy) {
-        tMin = fmaxG(vtMin.x, vtMin.y); tMax = fminG(vtMax.x, vtMax.y);
-        return tMax >= 0.0f && tMax >= tMin;
-    }
-    if (dt.y + dt.z > tyz && dt.z + dt.x > tzx && dt.x + dt.y > txy) {
-        tMin = fmaxG(fmaxG(vtMin.x, vtMin.y), vtMin.z);
-        tMax = fminG(fminG(vtMax.x, vtMax.y), vtMax.z);
-        return tMax >= 0.0f && tMax >= tMin;
-    }
-    return false;
-}
+ZGL_D RayPre prepareRay(const Ray& r) { return prepareRay(r.ori, r.dir); }
 
 // intersection.glsl:63-109.  Returns hit and writes t; u, v, det for the bin.
 ZGL_D bool intersectTriangle(vec3 a, vec3 b, vec3 c, vec3 o, vec3 d, float& t, float* uo = nullptr,
@@ -202,26 +124,33 @@ ZGL_D void loadTri(const DevScene& s, int prim, vec3& a, vec3& b, vec3& c)
 }
 
 // bvhHit / bvhTest (intersection.glsl:367-427): stackless walk over the table picked
-// by cubemapFace(-dir); a missed or farther box jumps to the row's miss link.
+// by cubemapFace(-dir); a missed or farther box jumps to the row's miss link.  Row
+// indices and miss links are absolute (table * bvhSize + row), so a row's address is
+// one shift and add.  A wave whose rays are all "general" (RayPre::special false) uses
+// the three-slab test alone, which gives the same result for those rays.
+ZGL_D const float4* rowPtr(const DevScene& s, int k) { return s.rows + (size_t)(uint32_t)k * 2; }
+
 template <bool ANY, bool C>
 ZGL_D int traverse(const DevScene& s, const RayPre& p, float& dist, Counters& cn)
 {
     int closest = -1;
-    const float4* rows = s.rows + (size_t)p.table * (size_t)s.bvhSize * 2;
-    const int size = s.bvhSize;
+    const bool anySpecial = __ballot(p.special) != 0ull;
     if (C) cn.traversals++;
-    int k = 0;
-    while (k != size) {
-        float4 lo = rows[(size_t)k * 2 + 0];
-        float4 hi = rows[(size_t)k * 2 + 1];
+    int k = p.table * s.bvhSize;
+    const int end = k + s.bvhSize;
+    while (k != end) {
+        const float4* row = rowPtr(s, k);
+        const float4 lo = row[0];
+        const float4 hi = row[1];
         if (C) { cn.rows++; cn.bytes += 36; }
         float boxDist;
-        bool bHit = boxHit(v3(lo.x, lo.y, lo.z), v3(lo.w, hi.x, hi.y), p, boxDist);
+        const vec3 bMin = v3(lo.x, lo.y, lo.z), bMax = v3(lo.w, hi.x, hi.y);
+        const bool bHit = anySpecial ? boxHit(bMin, bMax, p, boxDist) : boxHitGeneral(bMin, bMax, p, boxDist);
         if (!bHit || boxDist > dist) {
             k = __float_as_int(hi.w);
             continue;
         }
-        int prim = __float_as_int(hi.z);
+        const int prim = __float_as_int(hi.z);
         if (prim >= 0) {
             vec3 a, b, c;
             loadTri(s, prim, a, b, c);
@@ -238,34 +167,41 @@ ZGL_D int traverse(const DevScene& s, const RayPre& p, float& dist, Counters& cn
     return closest;
 }
 
-// Resumable traversal: the state of one bvhHit / bvhTest walk, advanced one row per
-// call.  Visiting the rows in the same order with the same tests as traverse() gives
-// the same result bit for bit; the persistent kernels interleave many such walks.
+// Resumable traversal: the state of one bvhHit / bvhTest walk, advanced by walkStep
+// one row (plus that row's triangle test) at a time.  The rows are visited in the same
+// order with the same tests as traverse(), so the result is the same bit for bit
+// however walks of different lanes are interleaved.
 struct Walk {
     RayPre p;
     float dist;
-    int k;
+    int k;     // absolute row index
+    int end;   // one past the table's last row
     int closest;
     bool any;
 };
-template <bool C> ZGL_D void walkStart(Walk& w, const Ray& r, bool any, float tmax, Counters& cn)
+template <bool C>
+ZGL_D void walkStart(const DevScene& s, Walk& w, const Ray& r, bool any, float tmax, Counters& cn)
 {
     if (C) cn.traversals++;
     w.p = prepareRay(r);
     w.dist = tmax;
-    w.k = 0;
+    w.k = w.p.table * s.bvhSize;
+    w.end = w.k + s.bvhSize;
     w.closest = -1;
     w.any = any;
 }
-// One row; returns true when the walk has finished.
-template <bool C> ZGL_D bool walkStep(const DevScene& s, Walk& w, Counters& cn)
+// One row (and its triangle when the row is a leaf whose box is hit); returns true
+// when the walk has finished.  `general` is true when the caller knows that no lane
+// walking with it has a special ray.
+template <bool C> ZGL_D bool walkStep(const DevScene& s, Walk& w, bool general, Counters& cn)
 {
-    const float4* rows = s.rows + ((size_t)w.p.table * (size_t)s.bvhSize + (size_t)w.k) * 2;
-    const float4 lo = rows[0];
-    const float4 hi = rows[1];
+    const float4* row = rowPtr(s, w.k);
+    const float4 lo = row[0];
+    const float4 hi = row[1];
     if (C) { cn.rows++; cn.bytes += 36; }
     float boxDist;
-    const bool bHit = boxHit(v3(lo.x, lo.y, lo.z), v3(lo.w, hi.x, hi.y), w.p, boxDist);
+    const vec3 bMin = v3(lo.x, lo.y, lo.z), bMax = v3(lo.w, hi.x, hi.y);
+    const bool bHit = general ? boxHitGeneral(bMin, bMax, w.p, boxDist) : boxHit(bMin, bMax, w.p, boxDist);
     if (!bHit || boxDist > w.dist) {
         w.k = __float_as_int(hi.w);
     } else {
@@ -283,7 +219,7 @@ template <bool C> ZGL_D bool walkStep(const DevScene& s, Walk& w, Counters& cn)
         }
         w.k++;
     }
-    return w.k == s.bvhSize;
+    return w.k == w.end;
 }
 
 template <bool C> ZGL_D int bvhHit(const DevScene& s, const Ray& r, float& dist, Counters& cn)

@@ -31,6 +31,9 @@ enum : int {
     kDone,
 };
 
+// MTBVH rows each walking lane advances between two ballots of the walking loop.
+constexpr int kRowsPerRound = 4;
+
 struct PixelSeed2 { uint32_t tx, ty; };
 
 __device__ __forceinline__ PixelSeed2 seedOf(const DevScene& s, vec2 scr)
@@ -64,7 +67,7 @@ template <bool C> __device__ void flushCountersP(const Counters& cn, unsigned lo
 }
 
 template <bool C>
-__global__ __launch_bounds__(256) void k_naive_persistent(DevScene s, PassParams prm, float4* __restrict__ frame,
+__global__ __launch_bounds__(256, 4) void k_naive_persistent(DevScene s, PassParams prm, float4* __restrict__ frame,
                                                           int* __restrict__ pixelCounter, int threshold,
                                                           unsigned long long* counters)
 {
@@ -124,7 +127,7 @@ __global__ __launch_bounds__(256) void k_naive_persistent(DevScene s, PassParams
                 }
                 if (terminate) phase = kFinish;
                 else {
-                    walkStart<C>(wk, rayOffseted(pos, wi), false, 1e8f, cn);
+                    walkStart<C>(s, wk, rayOffseted(pos, wi), false, 1e8f, cn);
                     walking = true;
                     phase = kExtend;
                 }
@@ -251,13 +254,13 @@ __global__ __launch_bounds__(256) void k_naive_persistent(DevScene s, PassParams
                     rrProb = fminG(maxComponent(bs.bsdf / bsdfPdf), 0.95f);
                 }
                 if (shadow) {
-                    walkStart<C>(wk, shadowRay, true, shadowDist, cn);
+                    walkStart<C>(s, wk, shadowRay, true, shadowDist, cn);
                     walking = true;
                     phase = kShadow;
                 } else if (terminate) {
                     phase = kFinish;
                 } else {
-                    walkStart<C>(wk, rayOffseted(pos, wi), false, 1e8f, cn);
+                    walkStart<C>(s, wk, rayOffseted(pos, wi), false, 1e8f, cn);
                     walking = true;
                     phase = kExtend;
                 }
@@ -318,7 +321,7 @@ __global__ __launch_bounds__(256) void k_naive_persistent(DevScene s, PassParams
             rng.dim = 0;
             if (C) cn.bytes += 64 + 32;
             const vec4 u = sample4D<C>(s, rng, cn);
-            walkStart<C>(wk, cameraSampleRay(s, scr, u), false, 1e8f, cn);
+            walkStart<C>(s, wk, cameraSampleRay(s, scr, u), false, 1e8f, cn);
             walking = true;
             phase = kPrimary;
         }
@@ -326,11 +329,17 @@ __global__ __launch_bounds__(256) void k_naive_persistent(DevScene s, PassParams
         if (__ballot(phase != kDone) == 0ull) break;
 
         // ------------------------------------------------------------ walking
-        for (;;) {
-            if (walking) {
-                if (walkStep<C>(s, wk, cn)) walking = false;
+        {
+            const bool general = __ballot(walking && wk.p.special) == 0ull;
+            for (;;) {
+#pragma unroll
+                for (int u = 0; u < kRowsPerRound; ++u) {
+                    if (walking) {
+                        if (walkStep<C>(s, wk, general, cn)) walking = false;
+                    }
+                }
+                if (__popcll(__ballot(walking)) <= threshold) break;
             }
-            if (__popcll(__ballot(walking)) <= threshold) break;
         }
     }
     flushCountersP<C>(cn, counters);
