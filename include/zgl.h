@@ -174,6 +174,29 @@ void* zgl_frame_device_ptr(zgl_context* ctx);
 zgl_status zgl_read_counters(zgl_context* ctx, uint64_t* out, int32_t n);
 zgl_status zgl_reset_counters(zgl_context* ctx);
 
+/* ---- Differentiable naive path tracer.  No counterpart in the reference (it has no
+ * differentiable mode); the estimator is path_integ_naive.glsl:35-174's.
+ *
+ * Parameters: per material, (baseColor r, g, b, roughness, metallic) as stored in the
+ * material record (slots 0, 1, 2, 3, 5; roughness before the loader's remap), used by
+ * principled and Lambertian materials; per environment texel, RGB in FP32.
+ * Sampling -- directions, lobe, light and texel choices, pdfs, MIS weights, Russian
+ * roulette -- stays at the uploaded scene.  BSDF values and environment radiance are
+ * taken at the evaluation point set here (NULL restores the uploaded values; at upload
+ * the evaluation point is the uploaded scene).  Values for materials of other types
+ * are ignored. */
+zgl_status zgl_set_eval_params(zgl_context* ctx, const float* material_params /* n_materials x 5 */,
+                               const float* env_rgb /* env_height x env_width x 3 */);
+/* The estimator at the evaluation point, added into the frame as zgl_render adds
+ * (naive integrator only).  At the uploaded parameters it is zgl_render's, bit for bit. */
+zgl_status zgl_render_replay(zgl_context* ctx, const zgl_pass_params* params);
+/* Path-replay adjoint: gradients of sum(d_frame . frame sum of these passes) with
+ * respect to the parameters at the evaluation point.  d_frame: W*H*4 floats (RGBA
+ * layout of zgl_read_frame, alpha ignored).  Outputs are overwritten: d_material_params
+ * n_materials x 5, d_env_rgb env_height x env_width x 3.  Does not change the frame. */
+zgl_status zgl_adjoint(zgl_context* ctx, const zgl_pass_params* params, const float* d_frame,
+                       float* d_material_params, float* d_env_rgb);
+
 #ifdef __cplusplus
 }
 #endif

@@ -13,6 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liborc.so")
 
 _fp = C.POINTER(C.c_float)
+_dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
 _up = C.POINTER(C.c_uint32)
 _u64 = C.POINTER(C.c_uint64)
@@ -73,6 +74,14 @@ def lib():
         L.orc_hash.restype = C.c_uint32
         L.orc_fmath.argtypes = [C.c_int, _fp, _fp, _fp, C.c_int64]
         L.orc_fmath.restype = None
+        L.orc_bsdf_eval_f64.argtypes = [_dp, C.c_int, _dp, _dp, _dp, _dp]
+        L.orc_render_replay.argtypes = [C.POINTER(OrcScene), C.POINTER(OrcParams), _fp, _fp, C.c_int, C.c_int,
+                                        C.c_int, C.c_int, _fp, C.c_int]
+        L.orc_adjoint.argtypes = [C.POINTER(OrcScene), C.POINTER(OrcParams), _fp, _fp, _fp, C.c_int, C.c_int,
+                                  C.c_int, C.c_int, _dp, _dp, C.c_int, C.c_int]
+        L.orc_replay_loss64.argtypes = [C.POINTER(OrcScene), C.POINTER(OrcParams), _fp, _fp, _fp, C.c_int,
+                                        C.c_int, C.c_int, C.c_int, C.c_int]
+        L.orc_replay_loss64.restype = C.c_double
         L.orc_box_hit.argtypes = [_fp, _fp, C.c_int64, C.POINTER(C.c_uint8), _fp]
         L.orc_trace_closest.argtypes = [C.POINTER(OrcScene), _fp, C.c_int64, _ip, _fp, _up, _u64, C.c_int]
         L.orc_render_pixels.argtypes = [C.POINTER(OrcScene), C.POINTER(OrcParams), C.c_int, C.c_int, C.c_int,
@@ -248,6 +257,63 @@ def bsdf_eval(mat_record, wo, wi, n, mode=0):
     lib().orc_bsdf_eval(_p(m, C.c_float), _p(a[0], C.c_float), _p(a[1], C.c_float), _p(a[2], C.c_float), mode,
                         _p(out, C.c_float))
     return out[:3], float(out[3])
+
+
+def bsdf_eval_f64(rec, mat_type, wo, wi, n):
+    """FP64 principled (1) / Lambertian (0) BSDF value from a double material record."""
+    a = [np.ascontiguousarray(x, np.float64) for x in (rec, wo, wi, n)]
+    out = np.zeros(3, np.float64)
+    lib().orc_bsdf_eval_f64(_p(a[0], C.c_double), mat_type, _p(a[1], C.c_double), _p(a[2], C.c_double),
+                            _p(a[3], C.c_double), _p(out, C.c_double))
+    return out
+
+
+def _region(ps, x0, y0, w, h):
+    return x0, y0, (ps.width if w is None else w), (ps.height if h is None else h)
+
+
+def render_replay(ps, prm, eval_mat, eval_env, x0=0, y0=0, w=None, h=None, nthreads=0):
+    """Frame sums (h, w, 3) of the differentiable naive estimator: sampling at the scene,
+    BSDFs of principled/Lambertian materials at the records eval_mat (n, 16) and
+    environment radiance from the FP32 texels eval_env (env_h, env_w, 3)."""
+    x0, y0, w, h = _region(ps, x0, y0, w, h)
+    em = np.ascontiguousarray(eval_mat, np.float32)
+    ee = np.ascontiguousarray(eval_env, np.float32)
+    out = np.zeros((h, w, 3), np.float32)
+    s = scene_struct(ps)
+    assert lib().orc_render_replay(C.byref(s), C.byref(prm), _p(em, C.c_float), _p(ee, C.c_float), x0, y0, w, h,
+                                   _p(out, C.c_float), nthreads) == 0
+    return out
+
+
+def adjoint(ps, prm, eval_mat, eval_env, d_frame, x0=0, y0=0, w=None, h=None, nthreads=0, fp32_values=False):
+    """FP64 gradient of sum(d_frame . frame) of render_replay w.r.t. the material
+    parameters (n, 5: baseColor rgb, raw roughness, metallic) and the env texels.
+    fp32_values=False: the exact gradient of replay_loss64 (BSDF values in FP64);
+    True: the gradient of the FP32 estimator (products use its FP32 BSDF values)."""
+    x0, y0, w, h = _region(ps, x0, y0, w, h)
+    em = np.ascontiguousarray(eval_mat, np.float32)
+    ee = np.ascontiguousarray(eval_env, np.float32)
+    df = np.ascontiguousarray(d_frame, np.float32)
+    assert df.shape == (h, w, 3)
+    dm = np.zeros((em.shape[0], 5), np.float64)
+    de = np.zeros(ee.shape, np.float64)
+    s = scene_struct(ps)
+    assert lib().orc_adjoint(C.byref(s), C.byref(prm), _p(em, C.c_float), _p(ee, C.c_float), _p(df, C.c_float),
+                             x0, y0, w, h, _p(dm, C.c_double), _p(de, C.c_double), nthreads,
+                             int(fp32_values)) == 0
+    return dm, de
+
+
+def replay_loss64(ps, prm, eval_mat, eval_env, d_frame, x0=0, y0=0, w=None, h=None, nthreads=0):
+    """sum(d_frame . frame) of render_replay evaluated in FP64 from the recorded paths."""
+    x0, y0, w, h = _region(ps, x0, y0, w, h)
+    em = np.ascontiguousarray(eval_mat, np.float32)
+    ee = np.ascontiguousarray(eval_env, np.float32)
+    df = np.ascontiguousarray(d_frame, np.float32)
+    s = scene_struct(ps)
+    return lib().orc_replay_loss64(C.byref(s), C.byref(prm), _p(em, C.c_float), _p(ee, C.c_float),
+                                   _p(df, C.c_float), x0, y0, w, h, nthreads)
 
 
 def bsdf_sample(mat_record, wo, n, u, mode=0, seed=1):

@@ -22,6 +22,7 @@
 //   reduction + truncated Taylor series (identical steps to the GPU code; pinned
 //   against libm double in the tests).
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <climits>
 #include <cmath>
@@ -530,6 +531,40 @@ static float halfToF(uint16_t h)
 }
 
 struct Cnt { uint64_t bytes = 0, rows = 0, tests = 0, trav = 0; };
+
+// Records of one sample of the differentiable naive estimator (replay with detached
+// sampling), kept by the oracle's FP64 adjoint.
+struct RepTap {
+    int idx[4];
+    double w[4];
+};
+struct RepVertex {
+    int mat;
+    uint32_t type;
+    bool diff;
+    float wo[3], wi[3], ns[3];
+    float pdf, cosf;  // BSDF sampling pdf (theta0) and the cosine factor (1 for delta)
+    float fConst[3];  // bs.f, the factor of a non-differentiable material
+    float fE[3];      // the FP32 BSDF value the estimator multiplied by
+    float rr;         // Russian-roulette division applied after this vertex (1 if none)
+};
+struct RepTerm {
+    int kind;  // 0 camera ray hits env, 1 NEE to a light, 2 NEE to env, 3 BSDF ray hits env, 4 ... hits a light
+    int v;     // NEE: the vertex; hits: the vertex whose BSDF ray hit
+    // NEE: the vertex's material and frame, the light direction and the BSDF value there
+    int mat;
+    uint32_t type;
+    bool diff;
+    float wo[3], ns[3], wl[3], f[3];
+    float satDot, weight, envPdf, sel;
+    float coef[3];  // NEE to a light: Le / pdf / sel (independent of the parameters)
+    float le[3];    // light hit: Le
+    RepTap tap;
+};
+struct RepSample {
+    std::vector<RepVertex> vs;
+    std::vector<RepTerm> ts;
+};
 
 template <class R> struct Ctx {
     using V = V3<R>;
@@ -1370,6 +1405,160 @@ template <class R> struct Ctx {
         return result;
     }
 
+    // ---------------------------------------------------------- differentiable naive
+    // The naive estimator with sampling at the scene (theta0) and BSDF values of
+    // differentiable materials (principled, Lambertian) and environment radiance at
+    // the evaluation point: material records `ce.s.materials`, FP32 env texels `ev`.
+    static bool differentiable(uint32_t t) { return t != 2 && t != 3 && t != 4; }
+    V envAtEval(W uv, const float* ev, RepTap* tap)
+    {
+        int Wd = s.env_width, H = s.env_height;
+        R u = uv.x * R(Wd) - R(0.5f), v = uv.y * R(H) - R(0.5f);
+        R fu = std::floor(u), fv = std::floor(v);
+        R al = u - fu, be = v - fv;
+        int i0 = wrap((int)fu, Wd), j0 = wrap((int)fv, H), i1 = wrap((int)fu + 1, Wd), j1 = wrap((int)fv + 1, H);
+        R w00 = (R(1) - al) * (R(1) - be), w10 = al * (R(1) - be), w01 = (R(1) - al) * be, w11 = al * be;
+        int id[4] = {j0 * Wd + i0, j0 * Wd + i1, j1 * Wd + i0, j1 * Wd + i1};
+        R ww[4] = {w00, w10, w01, w11};
+        if (tap)
+            for (int k = 0; k < 4; k++) { tap->idx[k] = id[k]; tap->w[k] = (double)ww[k]; }
+        auto T = [&](int k) { return V(R(ev[3 * (size_t)id[k]]), R(ev[3 * (size_t)id[k] + 1]), R(ev[3 * (size_t)id[k] + 2])); };
+        return ((T(0) * w00 + T(1) * w10) + T(2) * w01) + T(3) * w11;
+    }
+    V envLeEval(V wi, const float* ev, RepTap* tap) { return envAtEval(sphereToPlane(rotZ(wi, rotCN, rotSN)), ev, tap); }
+
+    V naiveReplay(V o, V d, Ctx& ce, const float* ev, RepSample* rec)
+    {
+        auto put3 = [](float* dst, V x) { dst[0] = (float)x.x; dst[1] = (float)x.y; dst[2] = (float)x.z; };
+        R primDist;
+        int id = bvhHit(o, d, primDist);
+        V pos = o + d * primDist;
+        if (id == -1) {
+            RepTerm t = {};
+            t.kind = 0;
+            V le = envLeEval(d, ev, &t.tap);
+            if (rec) rec->ts.push_back(t);
+            return le;
+        }
+        if (id - s.obj_prim_count >= 0) return lightLe(id - s.obj_prim_count, pos, -d);
+        V wo = -d, result(0), tp(1);
+        for (int bounce = 1; bounce <= p.max_depth; bounce++) {
+            Surf sf = surface(id, pos);
+            int mat = s.mat_tex_indices[id] & 0xffff;
+            uint32_t t = matType(mat);
+            if (t != 3 && t != 4 && dot(sf.ns, wo) < R(0)) { sf.ns = -sf.ns; sf.ng = -sf.ng; }
+            P mp = loadMat(t, mat);
+            const bool diff = differentiable(t);
+            P mpE = diff ? ce.loadMat(t, mat) : mp;
+            const int vi = rec ? (int)rec->vs.size() : 0;
+            if (p.sample_light) {
+                R ud = sample1D();
+                R u0 = sample1D(), u1 = sample1D(), u2 = sample1D(), u3 = sample1D();
+                R pl = pdfLightSel();
+                bool sl = ud < pl;
+                R sel = sl ? pl : R(1) - pl;
+                RepTerm term = {};
+                Li ls{V(0), V(0), R(0)};
+                R envPdf = 0;
+                if (sl) {
+                    ls = lightSampleLi(lightPick(W(u0, u1)), pos, W(u2, u3));
+                } else {
+                    const int w = s.env_width, h = s.env_height;
+                    int rx = (int)(R(h) * u0);
+                    int row = (u1 < envProbAt(w, rx)) ? rx : envAliasAt(w, rx);
+                    int cx = (int)(R(w) * u2);
+                    int col = (u3 < envProbAt(cx, row)) ? cx : envAliasAt(cx, row);
+                    W uv((R(col) + R(0.5f)) / R(w), (R(row) + R(0.5f)) / R(h));
+                    V we = rotZ(planeToSphere(uv), rotC, rotS);
+                    R pdf = envPortion(we) * R(w) * R(h) * R(0.5f) * sq(M<R>::PiInv);
+                    if (!bvhTest(pos + we * R(1e-4f), we, R(1e8f)) && pdf != R(0)) {
+                        envPdf = pdf;
+                        ls = Li{we, envLeEval(we, ev, &term.tap) / pdf, pdf};
+                    }
+                }
+                ls.coef = ls.coef / sel;
+                ls.pdf = ls.pdf * sel;
+                if (ls.pdf > R(0)) {
+                    V f = evalF(t, mpE, wo, ls.wi, sf.ns, 0);
+                    R fp = evalPdf(t, mp, wo, ls.wi, sf.ns);
+                    R wgt = ls.pdf * ls.pdf / (ls.pdf * ls.pdf + fp * fp);
+                    result = result + f * tp * satDot(sf.ns, ls.wi) * ls.coef * wgt;
+                    if (rec) {
+                        term.kind = sl ? 1 : 2;
+                        term.v = vi;
+                        term.mat = mat; term.type = t; term.diff = diff;
+                        put3(term.wo, wo); put3(term.ns, sf.ns); put3(term.f, f);
+                        put3(term.wl, ls.wi);
+                        term.satDot = (float)satDot(sf.ns, ls.wi);
+                        term.weight = (float)wgt;
+                        term.envPdf = (float)envPdf;
+                        term.sel = (float)sel;
+                        put3(term.coef, ls.coef);
+                        rec->ts.push_back(term);
+                    }
+                }
+            }
+            R a0 = sample1D(), a1 = sample1D(), a2 = sample1D();
+            S bs = sampleF(t, mp, sf.ns, wo, 0, V(a0, a1, a2));
+            bool delta = bs.flag == (1u << 3) || bs.flag == (1u << 4);
+            if (bs.pdf < R(1e-8f)) break;
+            V fE = diff ? evalF(t, mpE, wo, bs.wi, sf.ns, 0) : bs.f;
+            const R cosf = delta ? R(1) : absDot(sf.ns, bs.wi);
+            tp = tp * (fE / bs.pdf * cosf);
+            if (rec) {
+                RepVertex vx = {};
+                vx.mat = mat; vx.type = t; vx.diff = diff;
+                put3(vx.wo, wo); put3(vx.wi, bs.wi); put3(vx.ns, sf.ns);
+                vx.pdf = (float)bs.pdf; vx.cosf = (float)cosf;
+                put3(vx.fConst, bs.f);
+                put3(vx.fE, fE);
+                vx.rr = 1.0f;
+                rec->vs.push_back(vx);
+            }
+            V no = pos + bs.wi * R(1e-4f);
+            R dist;
+            int next = bvhHit(no, bs.wi, dist);
+            int lid = next - s.obj_prim_count;
+            V np = no + bs.wi * dist;
+            if (next == -1) {
+                RepTerm term = {};
+                V rad = envLeEval(bs.wi, ev, &term.tap);
+                R wgt = R(1);
+                if (p.sample_light && !delta) {
+                    R ep = envPdfLi(bs.wi) * pdfSelEnv();
+                    wgt = (ep <= R(0)) ? R(0) : bs.pdf * bs.pdf / (bs.pdf * bs.pdf + ep * ep);
+                }
+                result = result + rad * tp * wgt;
+                if (rec) { term.kind = 3; term.v = vi; term.weight = (float)wgt; rec->ts.push_back(term); }
+                break;
+            } else if (lid >= 0) {
+                V rad = lightLe(lid, np, -bs.wi);
+                R wgt = R(1);
+                if (p.sample_light && !delta) {
+                    R lp = lightPdfLi(lid, pos, np) * pdfSelLight(lid);
+                    wgt = (lp <= R(0)) ? R(0) : bs.pdf * bs.pdf / (bs.pdf * bs.pdf + lp * lp);
+                }
+                result = result + rad * tp * wgt;
+                if (rec) {
+                    RepTerm term = {};
+                    term.kind = 4; term.v = vi; term.weight = (float)wgt; put3(term.le, rad);
+                    rec->ts.push_back(term);
+                }
+                break;
+            }
+            if (p.russian_roulette) {
+                R cp = gmin(maxComp(bs.f / bs.pdf), R(0.95f));
+                if (sample1D() >= cp) break;
+                tp = tp / cp;
+                if (rec) rec->vs.back().rr = (float)cp;
+            }
+            id = next;
+            pos = np;
+            wo = -bs.wi;
+        }
+        return result;
+    }
+
     static R remap(R x) { return x < R(1e-8f) ? R(1) : x * x; }
 
     // triple_path_pass_pt.glsl:59-195
@@ -1714,6 +1903,215 @@ static void renderRegion(const orc_scene& s, const orc_params& p, int x0, int y0
     }
 }
 
+// ---------------------------------------------------------------- differentiable naive
+// FP64 value and Jacobian of a differentiable BSDF with respect to the five record
+// parameters (slots 0, 1, 2, 3, 5), by central differences of the FP64 evaluation.
+static const int kRepSlots[5] = {0, 1, 2, 3, 5};
+static V3<double> bsdf64At(const double* r, uint32_t t, V3<double> wo, V3<double> wi, V3<double> n)
+{
+    using D = Ctx<double>;
+    D::P p;
+    p.base = V3<double>(r[0], r[1], r[2]);
+    if (t == 1) {
+        p.rough = mixs(0.0134, 1.0, r[3]);
+        p.ss = r[4]; p.metal = r[5]; p.spec = r[6]; p.specTint = r[7];
+        p.sheen = r[8]; p.sheenTint = r[9]; p.cc = r[10]; p.ccGloss = r[11];
+    }
+    return D::evalF(t == 1 ? 1u : 0u, p, wo, wi, n, 0);
+}
+static void bsdf64(const float* rec, uint32_t t, const float* wo, const float* wi, const float* ns, double f[3],
+                   double J[3][5])
+{
+    double r[16];
+    for (int i = 0; i < 16; i++) r[i] = rec[i];
+    V3<double> WO(wo[0], wo[1], wo[2]), WI(wi[0], wi[1], wi[2]), N(ns[0], ns[1], ns[2]);
+    V3<double> v = bsdf64At(r, t, WO, WI, N);
+    f[0] = v.x; f[1] = v.y; f[2] = v.z;
+    for (int k = 0; k < 5; k++) {
+        const double h = 1e-6;
+        double rp[16], rm[16];
+        for (int i = 0; i < 16; i++) rp[i] = rm[i] = r[i];
+        rp[kRepSlots[k]] += h;
+        rm[kRepSlots[k]] -= h;
+        V3<double> a = bsdf64At(rp, t, WO, WI, N), b = bsdf64At(rm, t, WO, WI, N);
+        J[0][k] = (a.x - b.x) / (2 * h);
+        J[1][k] = (a.y - b.y) / (2 * h);
+        J[2][k] = (a.z - b.z) / (2 * h);
+    }
+}
+
+// Gradient of delta . C for one recorded sample, in FP64.  Each term of C is
+//   (product of the vertex factors before it) x (local factor),
+// a vertex factor being f / pdf * cos / rr, with f evaluated at the evaluation point
+// for differentiable materials; the gradient is taken factor by factor, with
+// df/dtheta from FP64 central differences.  fp32Values = false: f in FP64 (the exact
+// gradient of valueSample); true: f as the FP32 estimator computed it, so that the
+// result is the gradient of the FP32 estimator (FP32 and FP64 BSDF values differ
+// where the BSDF is ill-conditioned, e.g. pow5(1 - cos) for cos near 1).
+static void adjointSample(const float* evMat, const float* evEnv, const RepSample& rs, const double delta[3],
+                          double* dMat, double* dEnv, bool fp32Values)
+{
+    const int n = (int)rs.vs.size();
+    std::vector<std::array<double, 3>> g(n), gf(n);
+    std::vector<std::array<std::array<double, 5>, 3>> dg(n), dgf(n);
+    for (int j = 0; j < n; j++) {
+        const RepVertex& v = rs.vs[j];
+        double f[3], J[3][5] = {};
+        if (v.diff) bsdf64(evMat + 16 * (size_t)v.mat, v.type, v.wo, v.wi, v.ns, f, J);
+        else for (int c = 0; c < 3; c++) f[c] = v.fConst[c];
+        if (fp32Values) for (int c = 0; c < 3; c++) f[c] = v.fE[c];
+        for (int c = 0; c < 3; c++) {
+            g[j][c] = f[c] / v.pdf * v.cosf;
+            gf[j][c] = g[j][c] / v.rr;
+            for (int k = 0; k < 5; k++) {
+                dg[j][c][k] = J[c][k] / v.pdf * v.cosf;
+                dgf[j][c][k] = dg[j][c][k] / v.rr;
+            }
+        }
+    }
+    auto envLe64 = [&](const RepTap& t, int c) {
+        double x = 0;
+        for (int k = 0; k < 4; k++) x += t.w[k] * evEnv[3 * (size_t)t.idx[k] + c];
+        return x;
+    };
+    for (const RepTerm& t : rs.ts) {
+        const bool nee = t.kind == 1 || t.kind == 2;
+        const bool withG = t.kind == 3 || t.kind == 4;  // the hit vertex's own BSDF factor
+        const int m = (t.kind == 0) ? 0 : t.v;           // full factors 0..m-1
+        double fn[3] = {0, 0, 0}, Jn[3][5] = {};
+        if (nee) {
+            if (t.diff) bsdf64(evMat + 16 * (size_t)t.mat, t.type, t.wo, t.wl, t.ns, fn, Jn);
+            else for (int c = 0; c < 3; c++) fn[c] = t.f[c];
+            if (fp32Values) for (int c = 0; c < 3; c++) fn[c] = t.f[c];
+        }
+        for (int c = 0; c < 3; c++) {
+            // local factor and its derivative with respect to the env radiance of channel c
+            double rest = 0, local = 0, dLe = 0;
+            switch (t.kind) {
+            case 0: local = envLe64(t.tap, c); dLe = 1; break;
+            case 1: rest = (double)t.satDot * (double)t.coef[c] * t.weight; local = fn[c] * rest; break;
+            case 2:
+                rest = (double)t.satDot * envLe64(t.tap, c) / t.envPdf / t.sel * t.weight;
+                local = fn[c] * rest;
+                dLe = fn[c] * t.satDot / t.envPdf / t.sel * t.weight;
+                break;
+            case 3: local = envLe64(t.tap, c) * t.weight; dLe = t.weight; break;
+            default: local = (double)t.le[c] * t.weight; break;
+            }
+            const int nf = m + (withG ? 1 : 0);
+            auto factor = [&](int i) { return (withG && i == m) ? g[i][c] : gf[i][c]; };
+            double P = 1;
+            for (int i = 0; i < nf; i++) P *= factor(i);
+            const double wc = delta[c];
+            for (int i = 0; i < nf; i++) {
+                const RepVertex& v = rs.vs[i];
+                if (!v.diff) continue;
+                double ex = 1;
+                for (int l = 0; l < nf; l++) if (l != i) ex *= factor(l);
+                for (int k = 0; k < 5; k++) {
+                    const double d = (withG && i == m) ? dg[i][c][k] : dgf[i][c][k];
+                    dMat[5 * (size_t)v.mat + k] += wc * ex * d * local;
+                }
+            }
+            if (nee && t.diff)
+                for (int k = 0; k < 5; k++) dMat[5 * (size_t)t.mat + k] += wc * P * Jn[c][k] * rest;
+            if (t.kind == 0 || t.kind == 2 || t.kind == 3)
+                for (int q = 0; q < 4; q++)
+                    if (t.tap.w[q] != 0) dEnv[3 * (size_t)t.tap.idx[q] + c] += wc * P * dLe * t.tap.w[q];
+        }
+    }
+}
+
+// delta . C for one recorded sample, evaluated in FP64 from the records (the value whose
+// gradient adjointSample takes).
+static double valueSample(const float* evMat, const float* evEnv, const RepSample& rs, const double delta[3])
+{
+    const int n = (int)rs.vs.size();
+    std::vector<std::array<double, 3>> g(n), gf(n);
+    for (int j = 0; j < n; j++) {
+        const RepVertex& v = rs.vs[j];
+        double f[3], J[3][5];
+        if (v.diff) bsdf64(evMat + 16 * (size_t)v.mat, v.type, v.wo, v.wi, v.ns, f, J);
+        else for (int c = 0; c < 3; c++) f[c] = v.fConst[c];
+        for (int c = 0; c < 3; c++) { g[j][c] = f[c] / v.pdf * v.cosf; gf[j][c] = g[j][c] / v.rr; }
+    }
+    double sum = 0;
+    for (const RepTerm& t : rs.ts) {
+        double fn[3] = {0, 0, 0}, Jn[3][5];
+        const bool nee = t.kind == 1 || t.kind == 2;
+        if (nee) {
+            if (t.diff) bsdf64(evMat + 16 * (size_t)t.mat, t.type, t.wo, t.wl, t.ns, fn, Jn);
+            else for (int c = 0; c < 3; c++) fn[c] = t.f[c];
+        }
+        const int m = (t.kind == 0) ? 0 : t.v;
+        const bool withG = t.kind == 3 || t.kind == 4;
+        for (int c = 0; c < 3; c++) {
+            double le = 0;
+            if (t.kind == 0 || t.kind == 2 || t.kind == 3)
+                for (int k = 0; k < 4; k++) le += t.tap.w[k] * evEnv[3 * (size_t)t.tap.idx[k] + c];
+            double local = 0;
+            switch (t.kind) {
+            case 0: local = le; break;
+            case 1: local = fn[c] * ((double)t.satDot * (double)t.coef[c] * t.weight); break;
+            case 2: local = fn[c] * ((double)t.satDot * le / t.envPdf / t.sel * t.weight); break;
+            case 3: local = le * t.weight; break;
+            default: local = (double)t.le[c] * t.weight; break;
+            }
+            double P = 1;
+            for (int i = 0; i < m; i++) P *= gf[i][c];
+            if (withG) P *= g[m][c];
+            sum += delta[c] * P * local;
+        }
+    }
+    return sum;
+}
+
+// The replay estimator over an image region: frame sums (dFrame == null, FP32 like
+// renderRegion) or the FP64 gradient of sum(dFrame . frame) into dMat / dEnv.
+static void replayRegion(const orc_scene& s, const orc_params& p, const float* evMat, const float* evEnv, int x0,
+                         int y0, int w, int h, float* out, const float* dFrame, double* dMat, double* dEnv, Cnt& cn,
+                         int rowBegin, int rowEnd, double* loss = nullptr, bool fp32Values = false)
+{
+    const int samplerType = s.sampler ? 1 : 0;
+    orc_scene se = s;
+    se.materials = evMat;
+    for (int yy = rowBegin; yy < rowEnd; yy++) {
+        for (int xx = 0; xx < w; xx++) {
+            const int x = x0 + xx, y = y0 + yy;
+            float acc[3] = {0, 0, 0};
+            double delta[3] = {0, 0, 0};
+            if (dFrame)
+                for (int c = 0; c < 3; c++) delta[c] = dFrame[3 * ((size_t)yy * w + xx) + c];
+            for (int k = 0; k < p.n_passes; k++) {
+                const int spp = p.first_pass + k;
+                Ctx<float> c(s, p, cn, samplerType), ce(se, p, cn, samplerType);
+                c.seedPixel(x, y, spp, (uint32_t)(p.free_counter + k));
+                float u0 = c.sample1D(), u1 = c.sample1D(), u2 = c.sample1D(), u3 = c.sample1D();
+                V2<float> scr(float(x) / float(s.film_width), float(y) / float(s.film_height));
+                V3<float> o, d;
+                c.cameraRay(scr, u0, u1, u2, u3, o, d);
+                RepSample rs;
+                V3<float> r = c.naiveReplay(o, d, ce, evEnv, dFrame ? &rs : nullptr);
+                if (hasNan(r)) continue;
+                if (loss) {
+                    *loss += valueSample(evMat, evEnv, rs, delta);
+                } else if (dFrame) {
+                    adjointSample(evMat, evEnv, rs, delta, dMat, dEnv, fp32Values);
+                } else {
+                    const bool fresh = spp == 0;
+                    acc[0] = (fresh ? 0.0f : acc[0]) + r.x;
+                    acc[1] = (fresh ? 0.0f : acc[1]) + r.y;
+                    acc[2] = (fresh ? 0.0f : acc[2]) + r.z;
+                }
+            }
+            if (out) {
+                float* o3 = out + 3 * ((size_t)yy * w + xx);
+                o3[0] = acc[0]; o3[1] = acc[1]; o3[2] = acc[2];
+            }
+        }
+    }
+}
+
 template <class R>
 static void lightRange(const orc_scene& s, const orc_params& p, int64_t begin, int64_t end, std::vector<double>& film,
                        Cnt& cn, bool triple)
@@ -1881,6 +2279,76 @@ int orc_render_pixels(const orc_scene* s, const orc_params* p, int x0, int y0, i
     return 0;
 }
 
+// Differentiable naive estimator (no counterpart in the reference): the replay
+// estimator's frame sums over a region (FP32), and the FP64 gradient of
+// sum(d_frame . frame) w.r.t. the material parameters (n x 5: baseColor rgb, raw
+// roughness, metallic) and the FP32 env texels (h x w x 3).  eval_mat holds the
+// material records at the evaluation point, eval_env the env texels (FP32 RGB).
+// fp32_values selects the BSDF values of the products (see adjointSample).
+int orc_render_replay(const orc_scene* s, const orc_params* p, const float* eval_mat, const float* eval_env, int x0,
+                      int y0, int w, int h, float* out, int nthreads)
+{
+    if (!s || !p || !out || !eval_mat || !eval_env || w < 1 || h < 1) return 1;
+    const int T = std::min(threadsOf(nthreads), h);
+    std::vector<Cnt> cs(T);
+    std::vector<std::thread> th;
+    std::atomic<int> next(0);
+    for (int ti = 0; ti < T; ti++)
+        th.emplace_back([&, ti] {
+            for (int row; (row = next.fetch_add(1)) < h;)
+                replayRegion(*s, *p, eval_mat, eval_env, x0, y0, w, h, out, nullptr, nullptr, nullptr, cs[ti], row,
+                             row + 1);
+        });
+    for (auto& x : th) x.join();
+    return 0;
+}
+
+int orc_adjoint(const orc_scene* s, const orc_params* p, const float* eval_mat, const float* eval_env,
+                const float* d_frame, int x0, int y0, int w, int h, double* d_mat, double* d_env, int nthreads,
+                int fp32_values)
+{
+    if (!s || !p || !eval_mat || !eval_env || !d_frame || !d_mat || !d_env || w < 1 || h < 1) return 1;
+    const int T = std::min(threadsOf(nthreads), h);
+    const size_t nm = (size_t)s->n_materials * 5, ne = (size_t)s->env_width * s->env_height * 3;
+    std::vector<std::vector<double>> gm(T, std::vector<double>(nm, 0.0)), ge(T, std::vector<double>(ne, 0.0));
+    std::vector<Cnt> cs(T);
+    std::vector<std::thread> th;
+    std::atomic<int> next(0);
+    for (int ti = 0; ti < T; ti++)
+        th.emplace_back([&, ti] {
+            for (int row; (row = next.fetch_add(1)) < h;)
+                replayRegion(*s, *p, eval_mat, eval_env, x0, y0, w, h, nullptr, d_frame, gm[ti].data(),
+                             ge[ti].data(), cs[ti], row, row + 1, nullptr, fp32_values != 0);
+        });
+    for (auto& x : th) x.join();
+    for (size_t i = 0; i < nm; i++) { d_mat[i] = 0; for (int ti = 0; ti < T; ti++) d_mat[i] += gm[ti][i]; }
+    for (size_t i = 0; i < ne; i++) { d_env[i] = 0; for (int ti = 0; ti < T; ti++) d_env[i] += ge[ti][i]; }
+    return 0;
+}
+
+// sum(d_frame . frame) of the replay estimator evaluated in FP64 from the recorded
+// paths (the function orc_adjoint differentiates).
+double orc_replay_loss64(const orc_scene* s, const orc_params* p, const float* eval_mat, const float* eval_env,
+                         const float* d_frame, int x0, int y0, int w, int h, int nthreads)
+{
+    if (!s || !p || !eval_mat || !eval_env || !d_frame || w < 1 || h < 1) return 0.0;
+    const int T = std::min(threadsOf(nthreads), h);
+    std::vector<double> part(T, 0.0);
+    std::vector<Cnt> cs(T);
+    std::vector<std::thread> th;
+    std::atomic<int> next(0);
+    for (int ti = 0; ti < T; ti++)
+        th.emplace_back([&, ti] {
+            for (int row; (row = next.fetch_add(1)) < h;)
+                replayRegion(*s, *p, eval_mat, eval_env, x0, y0, w, h, nullptr, d_frame, nullptr, nullptr, cs[ti],
+                             row, row + 1, &part[ti]);
+        });
+    for (auto& x : th) x.join();
+    double sum = 0;
+    for (double v : part) sum += v;
+    return sum;
+}
+
 // Light-path integrators (light = 1, triple light pass = 2): splats of the paths
 // [path_begin, path_end) of the pass range into film (3W x H, accumulated in FP64).
 int orc_render_light(const orc_scene* s, const orc_params* p, int64_t path_begin, int64_t path_end, double* film,
@@ -1926,6 +2394,25 @@ int orc_bsdf_eval(const float* mat, const float* wo, const float* wi, const floa
     out[3] = Ctx<float>::evalPdf(t, mp, WO, WI, N);
     return 0;
 }
+// FP64 evaluation of the principled (type 1) or Lambertian (type 0) BSDF from a
+// material record given in double, the loader's roughness remap done in double:
+// the reference point for derivatives taken by central differences.
+int orc_bsdf_eval_f64(const double* rec, int type, const double* wo, const double* wi, const double* n, double* out)
+{
+    using D = Ctx<double>;
+    D::P p;
+    p.base = V3<double>(rec[0], rec[1], rec[2]);
+    if (type == 1) {
+        p.rough = mixs(0.0134, 1.0, rec[3]);
+        p.ss = rec[4]; p.metal = rec[5]; p.spec = rec[6]; p.specTint = rec[7];
+        p.sheen = rec[8]; p.sheenTint = rec[9]; p.cc = rec[10]; p.ccGloss = rec[11];
+    }
+    V3<double> WO(wo[0], wo[1], wo[2]), WI(wi[0], wi[1], wi[2]), N(n[0], n[1], n[2]);
+    V3<double> f = D::evalF(type == 1 ? 1u : 0u, p, WO, WI, N, 0);
+    out[0] = f.x; out[1] = f.y; out[2] = f.z;
+    return 0;
+}
+
 int orc_bsdf_sample(const float* mat, const float* wo, const float* n, int mode, const float* u, uint32_t rng_seed,
                     float* out)
 {

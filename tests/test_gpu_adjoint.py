@@ -1,0 +1,171 @@
+"""GPU parity of the differentiable naive estimator and its path-replay adjoint.
+
+  * at the uploaded parameters the replay kernel's frame equals zgl_render's bit for bit;
+  * at perturbed parameters it equals the oracle's FP32 replay (same operations), per
+    pixel within 1e-4 relative;
+  * the adjoint equals the oracle's FP64 adjoint of the FP32 estimator per gradient
+    component (components above 1e-3 of the largest): environment texels within 1e-3
+    relative, material parameters within 5e-3.  The oracle's BSDF derivatives are FP64
+    central differences, the kernel's are FP32 dual numbers; they agree to ~1e-5
+    except for the roughness of glossy lobes (raw roughness near the 0.05 floor of the
+    scene generator, alpha^2 ~ 1e-5), where (alpha^2 - 1) cos^2 + 1 loses 1e-3 of its
+    value to FP32 rounding;
+  * through the torch.autograd.Function, the gradient of an L2 image loss against a
+    target rendered at parameters perturbed by N(0, 0.1) (clamped to [0, 1]) agrees
+    with fourth-order central finite differences (h = 2^-10 ~ 1e-3) on 64 parameters:
+    at least 60 within 1e-3 relative, and every one within 1e-3 plus the differences'
+    own error estimate (the spread of the fourth-order and the second-order differences
+    at steps h and h/2), which is large only for the roughness of glossy lobes, where
+    FP32 rounding of (alpha^2 - 1) cos^2 + 1 blurs a 1e-3 step.
+"""
+import numpy as np
+import pytest
+import torch
+
+from zillumgl_amd import scene, render, diff
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+PASSES, DEPTH = 3, 4
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    print("[gpu] adjoint: creating context on device 0", flush=True)
+    c = render.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def c2s():
+    return scene.pack(scene.config2(width=96, height=54, n_spheres=6, subdiv=3, env_size=(256, 128), seed=11))
+
+
+def _records(ps, params):
+    em = ps.materials.copy()
+    em[:, 0], em[:, 1], em[:, 2], em[:, 3], em[:, 5] = (params[:, k] for k in range(5))
+    return em
+
+
+def _perturb(ps, seed):
+    """Parameters perturbed by N(0, 0.1) and clamped to the generator's ranges (raw
+    roughness [0.05, 1], others [0, 1]); env texels scaled by lognormal(0, 0.1)."""
+    rng = np.random.default_rng(seed)
+    p0, e0 = diff.scene_parameters(ps)
+    p1 = np.clip(p0 + rng.normal(0, 0.1, p0.shape), 0, 1).astype(np.float32)
+    p1[:, 3] = np.maximum(p1[:, 3], np.float32(0.05))
+    e1 = np.clip(e0 * rng.lognormal(0, 0.1, e0.shape), 0, None).astype(np.float32)
+    return p0, e0, p1, e1
+
+
+def test_replay_at_uploaded_parameters_equals_forward(ctx, c2s):
+    print("[gpu] adjoint: replay at uploaded parameters vs zgl_render", flush=True)
+    ctx.upload(c2s)
+    ctx.clear()
+    ctx.render(0, 0, PASSES, 1, DEPTH)
+    fwd = ctx.read_frame()[..., :3]
+    ctx.set_eval_params(None, None)
+    ctx.clear()
+    ctx.render_replay(0, PASSES, 1, DEPTH)
+    rep = ctx.read_frame()[..., :3]
+    assert np.array_equal(fwd.view(np.uint32), rep.view(np.uint32))
+    # setting the uploaded values explicitly gives the same frame
+    p0, e0 = diff.scene_parameters(c2s)
+    ctx.set_eval_params(p0, e0)
+    ctx.clear()
+    ctx.render_replay(0, PASSES, 1, DEPTH)
+    assert np.array_equal(fwd.view(np.uint32), ctx.read_frame()[..., :3].view(np.uint32))
+
+
+def test_replay_at_perturbed_parameters_matches_oracle(ctx, c2s):
+    print("[gpu] adjoint: replay at perturbed parameters vs oracle", flush=True)
+    ctx.upload(c2s)
+    _, _, p1, e1 = _perturb(c2s, 1)
+    ctx.set_eval_params(p1, e1)
+    ctx.clear()
+    ctx.render_replay(0, PASSES, 1, DEPTH)
+    gpu = ctx.read_frame()[..., :3]
+    ref = orc.render_replay(c2s, orc.params(0, 0, PASSES, 1, DEPTH), _records(c2s, p1), e1)
+    err = np.abs(gpu.astype(np.float64) - ref) / np.maximum(np.abs(ref), 1e-3)
+    print(f"[gpu] adjoint: replay max rel err {err.max():.3g}", flush=True)
+    assert err.max() <= 1e-4
+    assert not np.array_equal(gpu, orc.render_pixels(c2s, orc.params(0, 0, PASSES, 1, DEPTH))[0])
+
+
+def _close(g, ref, rel):
+    floor = 1e-3 * np.abs(ref).max()
+    m = np.abs(ref) > floor
+    err = np.abs(g.astype(np.float64) - ref) / np.maximum(np.abs(ref), floor)
+    return err[m].max(), m.sum()
+
+
+def test_adjoint_matches_oracle_fp64(ctx, c2s):
+    print("[gpu] adjoint: GPU adjoint vs oracle FP64 adjoint", flush=True)
+    ctx.upload(c2s)
+    rng = np.random.default_rng(2)
+    _, _, p1, e1 = _perturb(c2s, 2)
+    d_frame = rng.uniform(-1, 1, (c2s.height, c2s.width, 3)).astype(np.float32)
+    ctx.set_eval_params(p1, e1)
+    gm, ge = ctx.adjoint(d_frame, p1.shape[0], e1.shape, 0, PASSES, 1, DEPTH)
+    rm, re = orc.adjoint(c2s, orc.params(0, 0, PASSES, 1, DEPTH), _records(c2s, p1), e1, d_frame, fp32_values=True)
+    em, nm = _close(gm, rm, 1e-3)
+    ee, ne = _close(ge, re, 1e-3)
+    print(f"[gpu] adjoint: material max rel err {em:.3g} over {nm}, env {ee:.3g} over {ne}", flush=True)
+    assert nm >= 20 and ne >= 1000
+    assert em <= 5e-3 and ee <= 1e-3
+
+
+def test_autograd_l2_gradient_matches_central_differences(ctx, c2s):
+    print("[gpu] adjoint: torch autograd L2 loss vs central differences", flush=True)
+    ctx.upload(c2s)
+    rd = diff.Renderer(ctx, c2s, 16, DEPTH)
+    p0, e0, pt, et = _perturb(c2s, 5)
+    target = torch.from_numpy(rd.image(pt, et))
+    theta_m = torch.tensor(p0, requires_grad=True)
+    theta_e = torch.tensor(e0, requires_grad=True)
+    loss = ((diff.render_image(theta_m, theta_e, rd) - target) ** 2).sum()
+    loss.backward()
+    gm, ge = theta_m.grad.numpy(), theta_e.grad.numpy()
+
+    def loss_at(pm, pe):
+        img = rd.image(pm, pe).astype(np.float64)
+        return ((img - target.numpy().astype(np.float64)) ** 2).sum()
+
+    # 64 parameters: every material parameter with a gradient, then the largest env texels.
+    # Fourth-order central differences (Richardson on steps h and 2h, h = 2^-10 ~ 1e-3 of
+    # the parameter's scale): the O(h^2) term of the plain central difference is ~1e-3
+    # relative for the roughness of glossy lobes (D ~ roughness^-4 near the peak).
+    def fd(kind, i):
+        """Fourth-order central difference (steps h, 2h; h = 2^-10 of the parameter's
+        scale) and an estimate of its error: its distance to the second-order ones at
+        steps h and h/2."""
+        base = (p0 if kind == "m" else e0).ravel()
+        h = np.float32(2.0 ** -10 * max(1.0, abs(float(base[i]))))
+        vals = {}
+        for m in (-2, -1, -0.5, 0.5, 1, 2):
+            q = base.copy()
+            q[i] = np.float32(base[i] + np.float32(m) * h)
+            q = q.reshape(p0.shape if kind == "m" else e0.shape)
+            vals[m] = (loss_at(q, e0) if kind == "m" else loss_at(p0, q), float(q.ravel()[i]) - float(base[i]))
+        c = {m: (vals[m][0] - vals[-m][0]) / (vals[m][1] - vals[-m][1]) for m in (0.5, 1, 2)}
+        f4 = (4 * c[1] - c[2]) / 3
+        return f4, abs(f4 - c[1]) + abs(c[1] - c[0.5])
+
+    rng = np.random.default_rng(9)
+    mat_idx = [i for i in np.argsort(-np.abs(gm).ravel()) if abs(gm.ravel()[i]) > 1e-3 * np.abs(gm).max()][:24]
+    env_idx = list(np.argsort(-np.abs(ge).ravel())[:200])
+    env_idx = list(rng.choice(env_idx, 64 - len(mat_idx), replace=False))
+    strict, worst = 0, 0.0
+    for kind, i in [("m", i) for i in mat_idx] + [("e", i) for i in env_idx]:
+        f4, unc = fd(kind, i)
+        g = float((gm if kind == "m" else ge).ravel()[i])
+        rel = abs(g - f4) / max(abs(f4), 1e-12)
+        strict += rel <= 1e-3
+        worst = max(worst, rel)
+        # the differences' own error: FP32 rounding of the glossy lobes' (alpha^2 - 1) is
+        # ~1e-3 of the change a step of 1e-3 in raw roughness makes there
+        assert abs(g - f4) <= 1e-3 * abs(f4) + 2 * unc, (kind, i, g, f4, unc)
+    print(f"[gpu] adjoint: 64 central differences: {strict} within 1e-3, worst rel err {worst:.3g}", flush=True)
+    assert strict >= 60

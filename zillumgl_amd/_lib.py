@@ -89,6 +89,9 @@ SIGNATURES = [
     ("zgl_frame_device_ptr", C.c_void_p, [C.c_void_p]),
     ("zgl_read_counters", C.c_int32, [C.c_void_p, _u64p, C.c_int32]),
     ("zgl_reset_counters", C.c_int32, [C.c_void_p]),
+    ("zgl_set_eval_params", C.c_int32, [C.c_void_p, _fp, _fp]),
+    ("zgl_render_replay", C.c_int32, [C.c_void_p, C.POINTER(PassParams)]),
+    ("zgl_adjoint", C.c_int32, [C.c_void_p, C.POINTER(PassParams), _fp, _fp, _fp]),
 ]
 
 _lib = None

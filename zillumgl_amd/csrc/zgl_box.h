@@ -28,7 +28,7 @@ struct RayPre {
 ZGL_HD bool finite3(vec3 v)
 {
     const float big = 3.4028235e38f;  // FLT_MAX; false for +-inf and NaN
-    return (fabsG(v.x) <= big) & (fabsG(v.y) <= big) & (fabsG(v.z) <= big);
+    return fabsG(v.x) <= big && fabsG(v.y) <= big && fabsG(v.z) <= big;
 }
 
 ZGL_HD RayPre prepareRay(vec3 o, vec3 d)

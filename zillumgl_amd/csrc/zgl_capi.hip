@@ -1,4 +1,5 @@
 // zgl_capi.hip -- the C ABI: context, validated scene upload, pass dispatch.
+#include <cmath>
 #include <cstring>
 #include <new>
 #include <stdexcept>
@@ -48,6 +49,9 @@ struct zgl_context {
     DevBuf rows, tris, indices, vertices, normals, texcoords, matTex, materials;
     DevBuf lightPower, lightAlias, lightProb, env, envAlias, envProb, noise, sobolM, sobolTab;
     DevBuf frame, film, counters, scratchRays, scratchOut, workCounter;
+    // differentiable mode: evaluation point, adjoint input and outputs
+    DevBuf evalMat, evalEnv, dFrame, dMat, dEnv;
+    std::vector<float> hostMaterials;  // uploaded material records (16 floats each)
     int numCUs = 0;
     int naiveBlocksPerCU = 0;
     DevScene scene = {};
@@ -75,7 +79,8 @@ struct zgl_context {
     {
         DevBuf* all[] = {&rows, &tris, &indices, &vertices, &normals, &texcoords, &matTex, &materials,
                          &lightPower, &lightAlias, &lightProb, &env, &envAlias, &envProb, &noise, &sobolM,
-                         &sobolTab, &frame, &film, &counters, &scratchRays, &scratchOut, &workCounter};
+                         &sobolTab, &frame, &film, &counters, &scratchRays, &scratchOut, &workCounter,
+                         &evalMat, &evalEnv, &dFrame, &dMat, &dEnv};
         for (DevBuf* b : all) b->release();
     }
 };
@@ -356,6 +361,12 @@ zgl_status zgl_upload_scene(zgl_context* ctx, const zgl_scene_buffers* sb)
         sincosZ(sb->env_rotation, &s.envRotSin, &s.envRotCos);
         sincosZ(-sb->env_rotation, &s.envRotSinN, &s.envRotCosN);
         s.sampler = sb->sampler ? 1 : 0;
+        // evaluation point of the differentiable mode: the uploaded values
+        ctx->hostMaterials.assign(sb->materials, sb->materials + (size_t)sb->n_materials * 16);
+        ctx->upload(ctx->evalMat, sb->materials, (size_t)sb->n_materials * 64, "eval materials");
+        ctx->ensure(ctx->evalEnv, (size_t)w * h * 16, "eval env");
+        hipCheck(launchEnvExpand(s.env, (float4*)ctx->evalEnv.p, (int64_t)w * h, ctx->stream), "eval env");
+        hipCheck(hipStreamSynchronize(ctx->stream), "scene upload");
         ctx->hasScene = true;
     });
 }
@@ -480,46 +491,53 @@ zgl_status zgl_trace_any(zgl_context* ctx, const float* rays, const float* tmax,
 }
 
 // ---------------------------------------------------------------- rendering
+// Validates a pass request and builds the per-launch scene view and parameters.
+static void preparePasses(zgl_context* ctx, const zgl_pass_params* p, DevScene& s, PassParams& pp)
+{
+    if (!p) invalid("null params");
+    if (!ctx->hasScene || !ctx->hasCamera || !ctx->hasNoise)
+        throw ZglError{ZGL_E_STATE, "scene, camera and noise must be set before rendering"};
+    if (p->n_passes < 1 || p->n_passes > 65536 || p->first_pass < 0 || p->first_pass > (1 << 24) ||
+        p->max_depth < 0 || p->max_depth > 1024)
+        invalid("pass range / depth");
+    if (p->integrator < 0 || p->integrator > 2) invalid("integrator");
+    if ((p->integrator == ZGL_INTEGRATOR_LIGHT || p->integrator == ZGL_INTEGRATOR_TRIPLE) &&
+        ctx->scene.nLightTriangles < 1)
+        invalid("light and triple integrators need light triangles");
+    if (p->regen_threshold < 0 || p->regen_threshold > 63) invalid("regen_threshold must be in [0, 63]");
+    if (p->integrator != ZGL_INTEGRATOR_NAIVE && (p->blocks_one_pass < 1 || p->blocks_one_pass > 65536))
+        invalid("blocks_one_pass");
+    if (p->integrator == ZGL_INTEGRATOR_TRIPLE && (p->loops_per_pass < 1 || p->loops_per_pass > 1024))
+        invalid("loops_per_pass");
+    // Sobol points of the passes in this launch, plus one for dims spilling past 256.
+    const int first = p->first_pass, count = p->n_passes + 1;
+    ctx->ensure(ctx->sobolTab, (size_t)count * 256 * 4, "sobol table");
+    hipCheck(launchSobolTable((const uint32_t*)ctx->sobolM.p, (uint32_t*)ctx->sobolTab.p, first, count,
+                              ctx->stream),
+             "sobol table");
+    s = ctx->scene;
+    s.sobolTab = (const uint32_t*)ctx->sobolTab.p;
+    s.sobolTabFirst = first;
+    s.sobolTabCount = count;
+    pp.firstPass = p->first_pass;
+    pp.nPasses = p->n_passes;
+    pp.freeCounter = p->free_counter;
+    pp.maxDepth = p->max_depth;
+    pp.russianRoulette = p->russian_roulette;
+    pp.sampleLight = p->sample_light;
+    pp.lightEnvUniform = p->light_env_uniform;
+    pp.lightPortion = p->light_portion;
+    pp.loopsPerPass = p->loops_per_pass > 0 ? p->loops_per_pass : 1;
+    pp.pathsPerPass = 1536 * (p->blocks_one_pass > 0 ? p->blocks_one_pass : 1);
+    pp.lptScale = 0.0f;
+}
+
 zgl_status zgl_render(zgl_context* ctx, const zgl_pass_params* p)
 {
     return guarded(ctx, [&] {
-        if (!p) invalid("null params");
-        if (!ctx->hasScene || !ctx->hasCamera || !ctx->hasNoise)
-            throw ZglError{ZGL_E_STATE, "scene, camera and noise must be set before rendering"};
-        if (p->n_passes < 1 || p->n_passes > 65536 || p->first_pass < 0 || p->first_pass > (1 << 24) ||
-            p->max_depth < 0 || p->max_depth > 1024)
-            invalid("pass range / depth");
-        if (p->integrator < 0 || p->integrator > 2) invalid("integrator");
-        if ((p->integrator == ZGL_INTEGRATOR_LIGHT || p->integrator == ZGL_INTEGRATOR_TRIPLE) &&
-            ctx->scene.nLightTriangles < 1)
-            invalid("light and triple integrators need light triangles");
-        if (p->regen_threshold < 0 || p->regen_threshold > 63) invalid("regen_threshold must be in [0, 63]");
-        if (p->integrator != ZGL_INTEGRATOR_NAIVE && (p->blocks_one_pass < 1 || p->blocks_one_pass > 65536))
-            invalid("blocks_one_pass");
-        if (p->integrator == ZGL_INTEGRATOR_TRIPLE && (p->loops_per_pass < 1 || p->loops_per_pass > 1024))
-            invalid("loops_per_pass");
-        // Sobol points of the passes in this launch, plus one for dims spilling past 256.
-        const int first = p->first_pass, count = p->n_passes + 1;
-        ctx->ensure(ctx->sobolTab, (size_t)count * 256 * 4, "sobol table");
-        hipCheck(launchSobolTable((const uint32_t*)ctx->sobolM.p, (uint32_t*)ctx->sobolTab.p, first, count,
-                                  ctx->stream),
-                 "sobol table");
-        DevScene s = ctx->scene;
-        s.sobolTab = (const uint32_t*)ctx->sobolTab.p;
-        s.sobolTabFirst = first;
-        s.sobolTabCount = count;
+        DevScene s;
         PassParams pp;
-        pp.firstPass = p->first_pass;
-        pp.nPasses = p->n_passes;
-        pp.freeCounter = p->free_counter;
-        pp.maxDepth = p->max_depth;
-        pp.russianRoulette = p->russian_roulette;
-        pp.sampleLight = p->sample_light;
-        pp.lightEnvUniform = p->light_env_uniform;
-        pp.lightPortion = p->light_portion;
-        pp.loopsPerPass = p->loops_per_pass > 0 ? p->loops_per_pass : 1;
-        pp.pathsPerPass = 1536 * (p->blocks_one_pass > 0 ? p->blocks_one_pass : 1);
-        pp.lptScale = 0.0f;
+        preparePasses(ctx, p, s, pp);
         unsigned long long* cnt = p->count_bytes ? (unsigned long long*)ctx->counters.p : nullptr;
         float4* frame = (float4*)ctx->frame.p;
         float* film = (float*)ctx->film.p;
@@ -542,6 +560,79 @@ zgl_status zgl_render(zgl_context* ctx, const zgl_pass_params* p)
             hipCheck(launchTripleLPT(sl, pp, film, cnt, ctx->stream), "triple light pass");
             hipCheck(launchFilmToFrame(film, frame, ctx->filmW, ctx->filmH, ctx->stream), "film copy");
         }
+    });
+}
+
+// ---------------------------------------------------------------- differentiable mode
+zgl_status zgl_set_eval_params(zgl_context* ctx, const float* material_params, const float* env_rgb)
+{
+    return guarded(ctx, [&] {
+        if (!ctx->hasScene) throw ZglError{ZGL_E_STATE, "no scene uploaded"};
+        const DevScene& s = ctx->scene;
+        std::vector<float> rec(ctx->hostMaterials);
+        if (material_params) {
+            for (int m = 0; m < s.nMaterials; m++) {
+                const float* q = material_params + (size_t)m * 5;
+                for (int k = 0; k < 5; k++)
+                    if (!(std::fabs(q[k]) <= 3.4028235e38f)) invalid("non-finite material parameter");
+                float* r = rec.data() + (size_t)m * 16;
+                r[0] = q[0]; r[1] = q[1]; r[2] = q[2]; r[3] = q[3]; r[5] = q[4];
+            }
+        }
+        ctx->upload(ctx->evalMat, rec.data(), rec.size() * 4, "eval materials");
+        const int64_t n = (int64_t)s.envW * s.envH;
+        if (env_rgb) {
+            std::vector<float> env((size_t)n * 4, 0.0f);
+            for (int64_t i = 0; i < n; i++)
+                for (int c = 0; c < 3; c++) env[i * 4 + c] = env_rgb[i * 3 + c];
+            ctx->upload(ctx->evalEnv, env.data(), env.size() * 4, "eval env");
+        } else {
+            hipCheck(launchEnvExpand(s.env, (float4*)ctx->evalEnv.p, n, ctx->stream), "eval env");
+        }
+        hipCheck(hipStreamSynchronize(ctx->stream), "eval params");
+    });
+}
+
+zgl_status zgl_render_replay(zgl_context* ctx, const zgl_pass_params* p)
+{
+    return guarded(ctx, [&] {
+        DevScene s;
+        PassParams pp;
+        preparePasses(ctx, p, s, pp);
+        if (p->integrator != ZGL_INTEGRATOR_NAIVE) invalid("the replay estimator is the naive integrator's");
+        hipCheck(launchNaiveReplay(s, pp, (const float4*)ctx->evalMat.p, (const float4*)ctx->evalEnv.p,
+                                   (float4*)ctx->frame.p, nullptr, nullptr, nullptr, ctx->stream),
+                 "replay pass");
+    });
+}
+
+zgl_status zgl_adjoint(zgl_context* ctx, const zgl_pass_params* p, const float* d_frame, float* d_material_params,
+                       float* d_env_rgb)
+{
+    return guarded(ctx, [&] {
+        DevScene s;
+        PassParams pp;
+        preparePasses(ctx, p, s, pp);
+        if (p->integrator != ZGL_INTEGRATOR_NAIVE) invalid("the adjoint is the naive integrator's");
+        if (!d_frame || !d_material_params || !d_env_rgb) invalid("null adjoint buffer");
+        const size_t px = (size_t)ctx->filmW * ctx->filmH;
+        const size_t nm = (size_t)s.nMaterials * 5, ne = (size_t)s.envW * s.envH * 3;
+        ctx->upload(ctx->dFrame, d_frame, px * 16, "adjoint input");
+        // FP64 accumulators, zeroed as float pairs (all-zero bits are 0.0 in both)
+        ctx->ensure(ctx->dMat, nm * 8, "material gradients");
+        ctx->ensure(ctx->dEnv, ne * 8, "environment gradients");
+        hipCheck(launchFill((float*)ctx->dMat.p, (int64_t)nm * 2, 0.0f, ctx->stream), "clear");
+        hipCheck(launchFill((float*)ctx->dEnv.p, (int64_t)ne * 2, 0.0f, ctx->stream), "clear");
+        hipCheck(launchNaiveReplay(s, pp, (const float4*)ctx->evalMat.p, (const float4*)ctx->evalEnv.p,
+                                   (float4*)ctx->frame.p, (const float4*)ctx->dFrame.p, (double*)ctx->dMat.p,
+                                   (double*)ctx->dEnv.p, ctx->stream),
+                 "adjoint pass");
+        std::vector<double> gm(nm), ge(ne);
+        hipCheck(hipMemcpyAsync(gm.data(), ctx->dMat.p, nm * 8, hipMemcpyDeviceToHost, ctx->stream), "copy");
+        hipCheck(hipMemcpyAsync(ge.data(), ctx->dEnv.p, ne * 8, hipMemcpyDeviceToHost, ctx->stream), "copy");
+        hipCheck(hipStreamSynchronize(ctx->stream), "adjoint");
+        for (size_t i = 0; i < nm; i++) d_material_params[i] = (float)gm[i];
+        for (size_t i = 0; i < ne; i++) d_env_rgb[i] = (float)ge[i];
     });
 }
 

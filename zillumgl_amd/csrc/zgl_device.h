@@ -1027,6 +1027,23 @@ ZGL_D Ray cameraSampleRay(const DevScene& s, vec2 uv, vec4 u)
     r.dir = dir;
     return r;
 }
+
+// Per-pixel seeds (path_integ_naive.glsl:152-164): a double bilinear lookup into
+// the noise texture picks the scramble and the rand() seed.
+struct PixelSeed { uint32_t tx, ty; };
+ZGL_D PixelSeed pixelSeed(const DevScene& s, vec2 scr)
+{
+    vec2 nc = noiseLookup(s, scr);
+    nc = noiseLookup(s, nc);
+    vec2 tc = v2((float)s.filmW, (float)s.filmH) * nc;
+    return PixelSeed{(uint32_t)tc.x, (uint32_t)tc.y};
+}
+ZGL_D int sampleOffsetOf(int spp)
+{
+    int off = spp * SobolSampleDim;
+    if (off > SobolSampleNum * SobolSampleDim) off -= SobolSampleNum * SobolSampleDim;
+    return off;
+}
 ZGL_D bool inFilmBound(vec2 uv) { return uv.x >= 0.0f && uv.x <= 1.0f && uv.y >= 0.0f && uv.y <= 1.0f; }
 ZGL_D bool cameraDelta(const DevScene& s) { return s.lensRadius <= 1e-6f; }
 ZGL_D vec2 cameraRasterPos(const DevScene& s, const Ray& ray)

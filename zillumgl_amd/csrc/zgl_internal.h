@@ -39,6 +39,11 @@ hipError_t launchTraceAny(const DevScene& s, const float* rays, const float* tma
 hipError_t launchNaivePersistent(const DevScene& s, const PassParams& p, float4* frame, int* pixelCounter,
                                  int threshold, int blocks, unsigned long long* counters, hipStream_t st);
 int naivePersistentBlocksPerCU();
+// Differentiable naive estimator: dFrame == null renders the replay estimator into
+// frame; otherwise accumulates FP64 gradients into dMat (nMaterials x 5) and dEnv.
+hipError_t launchNaiveReplay(const DevScene& s, const PassParams& p, const float4* evalMat, const float4* evalEnv,
+                             float4* frame, const float4* dFrame, double* dMat, double* dEnv, hipStream_t st);
+hipError_t launchEnvExpand(const ushort4* env, float4* out, int64_t n, hipStream_t st);
 hipError_t launchLight(const DevScene& s, const PassParams& p, float* film, unsigned long long* counters,
                        hipStream_t st);
 hipError_t launchTriplePT(const DevScene& s, const PassParams& p, float* film, unsigned long long* counters,

@@ -125,23 +125,6 @@ __global__ __launch_bounds__(kBlock) void k_trace_any(DevScene s, const float* _
     occ[i] = bvhTest<false>(s, r, tmax[i], cn) ? 1 : 0;
 }
 
-// Per-pixel seeds (path_integ_naive.glsl:152-164): a double bilinear lookup into
-// the noise texture picks the scramble and the rand() seed.
-struct PixelSeed { uint32_t tx, ty; };
-__device__ __forceinline__ PixelSeed pixelSeed(const DevScene& s, vec2 scr)
-{
-    vec2 nc = noiseLookup(s, scr);
-    nc = noiseLookup(s, nc);
-    vec2 tc = v2((float)s.filmW, (float)s.filmH) * nc;
-    return PixelSeed{(uint32_t)tc.x, (uint32_t)tc.y};
-}
-__device__ __forceinline__ int sampleOffsetOf(int spp)
-{
-    int off = spp * SobolSampleDim;
-    if (off > SobolSampleNum * SobolSampleDim) off -= SobolSampleNum * SobolSampleDim;
-    return off;
-}
-
 // ---------------------------------------------------------------------------
 // Film splats (light_path_integ.glsl:34-43); the film is planar R32F of (3W) x H.
 // ---------------------------------------------------------------------------

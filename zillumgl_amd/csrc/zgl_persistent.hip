@@ -34,16 +34,6 @@ enum : int {
 // MTBVH rows each walking lane advances between two ballots of the walking loop.
 constexpr int kRowsPerRound = 4;
 
-struct PixelSeed2 { uint32_t tx, ty; };
-
-__device__ __forceinline__ PixelSeed2 seedOf(const DevScene& s, vec2 scr)
-{
-    vec2 nc = noiseLookup(s, scr);
-    nc = noiseLookup(s, nc);
-    vec2 tc = v2((float)s.filmW, (float)s.filmH) * nc;
-    return PixelSeed2{(uint32_t)tc.x, (uint32_t)tc.y};
-}
-
 // Pixel order for the dynamic fetch: 8x8 tiles in row-major tile order.
 __device__ __forceinline__ void pixelOfIndex(int idx, int W, int& x, int& y)
 {
@@ -84,7 +74,7 @@ __global__ __launch_bounds__(256, 4) void k_naive_persistent(DevScene s, PassPar
     float4 mem = make_float4(0.f, 0.f, 0.f, 0.f);
     bool stored = false;
     vec2 scr = v2(0.f, 0.f);
-    PixelSeed2 ps = {0u, 0u};
+    PixelSeed ps = {0u, 0u};
     Rng rng = {0u, 0u, 0, 0};
     vec3 result = v3(0.f), throughput = v3(1.f), wo = v3(0.f), pos = v3(0.f), wi = v3(0.f);
     int bounce = 0;
@@ -301,7 +291,7 @@ __global__ __launch_bounds__(256, 4) void k_naive_persistent(DevScene s, PassPar
                     if (x < W && y < H) {  // slots of partial edge tiles fetch again next round
                         pix = y * W + x;
                         scr = v2((float)x, (float)y) / v2((float)W, (float)H);
-                        ps = seedOf(s, scr);
+                        ps = pixelSeed(s, scr);
                         mem = frame[pix];
                         pass = 0;
                         phase = kNewSample;
@@ -315,9 +305,7 @@ __global__ __launch_bounds__(256, 4) void k_naive_persistent(DevScene s, PassPar
             const uint32_t fc = (uint32_t)(prm.freeCounter + pass);
             rng.randSeed = ps.tx * fc + ps.ty;
             rng.sampleSeed = ps.tx * ps.ty;
-            int off = spp * SobolSampleDim;
-            if (off > SobolSampleNum * SobolSampleDim) off -= SobolSampleNum * SobolSampleDim;
-            rng.sampleOffset = off;
+            rng.sampleOffset = sampleOffsetOf(spp);
             rng.dim = 0;
             if (C) cn.bytes += 64 + 32;
             const vec4 u = sample4D<C>(s, rng, cn);

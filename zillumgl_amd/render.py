@@ -96,6 +96,41 @@ class Context:
                             loops_per_pass, int(count_bytes), regen_threshold)
         self._check(_lib.lib().zgl_render(self.h, C.byref(p)))
 
+    # ------------------------------------------------------------ differentiable mode
+    def _params(self, first_pass, n_passes, free_counter, max_depth, russian_roulette, sample_light,
+                light_env_uniform, light_portion):
+        return _lib.PassParams(_lib.INTEGRATOR_NAIVE, first_pass, n_passes, free_counter, max_depth,
+                               int(russian_roulette), int(sample_light), int(light_env_uniform), light_portion, 32, 1,
+                               0, 0)
+
+    def set_eval_params(self, material_params=None, env_rgb=None):
+        """Evaluation point of the differentiable estimator: material_params (n, 5) =
+        (baseColor rgb, raw roughness, metallic), env_rgb (env_h, env_w, 3); None keeps
+        the uploaded values."""
+        m = None if material_params is None else np.ascontiguousarray(material_params, np.float32)
+        e = None if env_rgb is None else np.ascontiguousarray(env_rgb, np.float32)
+        self._check(_lib.lib().zgl_set_eval_params(self.h, ptr(m, C.c_float), ptr(e, C.c_float)))
+
+    def render_replay(self, first_pass, n_passes, free_counter, max_depth, russian_roulette=False,
+                      sample_light=True, light_env_uniform=False, light_portion=0.5):
+        p = self._params(first_pass, n_passes, free_counter, max_depth, russian_roulette, sample_light,
+                         light_env_uniform, light_portion)
+        self._check(_lib.lib().zgl_render_replay(self.h, C.byref(p)))
+
+    def adjoint(self, d_frame, n_materials, env_shape, first_pass, n_passes, free_counter, max_depth,
+                russian_roulette=False, sample_light=True, light_env_uniform=False, light_portion=0.5):
+        """Gradients of sum(d_frame . frame) (d_frame (H, W, 3) or (H, W, 4)) w.r.t. the
+        material parameters (n, 5) and the env texels env_shape + (3,)."""
+        d = np.zeros((self.height, self.width, 4), np.float32)
+        d[..., :3] = np.asarray(d_frame, np.float32)[..., :3]
+        gm = np.zeros((n_materials, 5), np.float32)
+        ge = np.zeros(tuple(env_shape[:2]) + (3,), np.float32)
+        p = self._params(first_pass, n_passes, free_counter, max_depth, russian_roulette, sample_light,
+                         light_env_uniform, light_portion)
+        self._check(_lib.lib().zgl_adjoint(self.h, C.byref(p), ptr(d, C.c_float), ptr(gm, C.c_float),
+                                           ptr(ge, C.c_float)))
+        return gm, ge
+
     def clear(self):
         self._check(_lib.lib().zgl_clear_frame(self.h))
 
