@@ -84,6 +84,146 @@ def cpu_baseline(ps, depth, budget_s=20.0):
                       f"{dt:.1f} s, FP32 oracle (oracle/zgl_oracle.cpp), {threads} threads"}
 
 
+def adjoint_bench(hip, steps):
+    """Path-replay adjoint on the differentiable-rendering workload: the tessellated
+    Cornell box (201,588 triangles, 32 Disney materials, 256 x 128 sky environment),
+    512 x 512, 32 spp, depth 6, gradients of an L2 image loss against a target rendered
+    at parameters perturbed by N(0, 0.1) (clamped to [0, 1]) w.r.t. 160 material
+    parameters and 98,304 environment texel channels.  t_forward is zgl_render of the
+    same passes; t_adjoint is zgl_adjoint (its own forward replay, the adjoint replay,
+    the upload of dLoss/dframe and the download of the gradients)."""
+    import numpy as np
+    from zillumgl_amd import scene, render, diff, _lib
+    ps = scene.pack(scene.config5())
+    ctx = render.Context(0)
+    ctx.upload(ps)
+    stream = ctx.stream()
+    W, H, spp, depth = ps.width, ps.height, 32, 6
+    rng = np.random.default_rng(5)
+    p0, e0 = diff.scene_parameters(ps)
+    pt = np.clip(p0 + rng.normal(0, 0.1, p0.shape), 0, 1).astype(np.float32)
+    rd = diff.Renderer(ctx, ps, spp, depth)
+    target = rd.image(pt, e0)
+    img = rd.image(p0, e0)
+    d_image = 2.0 * (img - target)
+
+    def timed(fn):
+        ms = []
+        for _ in range(steps + 1):
+            a, b = hip.event(), hip.event()
+            hip.record(a, stream)
+            fn()
+            hip.record(b, stream)
+            ms.append(hip.elapsed_ms(a, b))
+        return sorted(ms[1:])[len(ms[1:]) // 2]
+
+    t_fwd = timed(lambda: ctx.render(_lib.INTEGRATOR_NAIVE, 0, spp, 1, depth))
+    gm = ge = None
+
+    def adj():
+        nonlocal gm, ge
+        gm, ge = rd.gradients(p0, e0, d_image)
+    t_adj = timed(adj)
+    ctx.close()
+    samples = W * H * spp
+    return {"metric": "adjoint-gradient Msamples/s", "value": round(samples / (t_adj / 1e3) / 1e6, 3),
+            "unit": "Msamples/s", "t_adjoint_ms": round(t_adj, 3), "t_forward_ms": round(t_fwd, 3),
+            "adjoint_over_forward": round(t_adj / t_fwd, 3),
+            "gradient_norms": {"materials": float(np.linalg.norm(gm)), "env": float(np.linalg.norm(ge))},
+            "workload": "tessellated Cornell box, 201,588 tris, 32 Disney materials, 256x128 env, 512x512, "
+                        "32 spp, depth 6, L2 loss vs a target at parameters + N(0, 0.1); 160 material + "
+                        "98,304 env parameters"}
+
+
+def batch_bench(reps):
+    """64 scenes (Cornell boxes tessellated to 1k-2M triangles, 128^2-1024^2, 4-64 spp,
+    depth 3-10) rendered (a) by 64 sequential per-scene zgl_render launches and (b) by
+    one batched persistent launch replayed from a HIP graph, after the autotuner picked
+    its scene order, regeneration threshold and occupancy.  Frames are bit-identical
+    between (a) and (b) (tests/test_gpu_batch.py), so RMSE is equal."""
+    from concurrent.futures import ThreadPoolExecutor
+    from zillumgl_amd import scene, render, _lib
+    descs = scene.config4()
+    t0 = time.time()
+    with ThreadPoolExecutor(16) as ex:
+        packed = list(ex.map(lambda d: scene.pack(d[0]), descs))
+    print(f"[bench] batch: 64 scenes packed in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    ctxs = []
+    for ps in packed:
+        c = render.Context(0)
+        c.upload(ps)
+        ctxs.append(c)
+    passes = [(0, spp, 1, depth) for (_, spp, depth) in descs]
+    samples = sum(d.width * d.height * spp for (d, spp, _) in descs)
+    tris = sum(int(ps.indices.shape[0]) for ps in packed)
+
+    def sequential():
+        for c, (first, n, fc, depth) in zip(ctxs, passes):
+            c.render(_lib.INTEGRATOR_NAIVE, first, n, fc, depth)
+            c.synchronize()
+    sequential()
+    seq = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        sequential()
+        seq.append(time.perf_counter() - t)
+    t_seq = sorted(seq)[len(seq) // 2]
+    b = render.Batch(0)
+    b.set(ctxs, passes)
+    t = time.perf_counter()
+    b.tune(reps=1)
+    t_tune = time.perf_counter() - t
+    cfg = b.config()
+    b.launch()
+    b.synchronize()
+    bat = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        b.launch()
+        b.synchronize()
+        bat.append(time.perf_counter() - t)
+    t_bat = sorted(bat)[len(bat) // 2]
+    b.close()
+    for c in ctxs:
+        c.close()
+    return {"metric": "batched Msamples/s", "value": round(samples / t_bat / 1e6, 3), "unit": "Msamples/s",
+            "t_batch_ms": round(t_bat * 1e3, 3), "t_sequential_ms": round(t_seq * 1e3, 3),
+            "sequential_value": round(samples / t_seq / 1e6, 3), "speedup_vs_sequential": round(t_seq / t_bat, 3),
+            "tuned": {"regen_threshold": cfg["regen_threshold"], "blocks_per_cu": cfg["blocks_per_cu"],
+                      "order_is_work_descending": bool(list(cfg["order"]) != list(range(len(ctxs)))),
+                      "tuning_s": round(t_tune, 1)},
+            "workload": f"64 Cornell-box scenes, {tris:,} triangles in total (1k-2M each, log-uniform), "
+                        f"128^2-1024^2, spp 4-64, depth 3-10, {samples / 1e6:.1f} M samples per launch; "
+                        "frames bit-identical between the two forms"}
+
+
+def rmse_vs_converged(ctx, W, H, spp, depth, ref_spp):
+    """RMSE of the benchmark image (mean of passes 0..spp-1) against a converged image of
+    ref_spp further passes (spp..spp+ref_spp-1, independent of the image's samples) of
+    the same estimator, summed per launch of 64 passes into FP64 on the host (each
+    launch's frame starts from zero).  The reference's own noise adds spp / ref_spp of
+    the image's variance to the squared RMSE; the value is corrected for it."""
+    import numpy as np
+    from zillumgl_amd import _lib
+    ctx.clear()
+    ctx.render(_lib.INTEGRATOR_NAIVE, 0, spp, 1, depth)
+    img = ctx.read_frame()[..., :3].astype(np.float64) / spp
+    acc = np.zeros((H, W, 3), np.float64)
+    chunk = 64
+    for first in range(spp, spp + ref_spp, chunk):  # passes disjoint from the image's
+        ctx.clear()
+        ctx.render(_lib.INTEGRATOR_NAIVE, first, chunk, 1 + first, depth)
+        acc += ctx.read_frame()[..., :3]
+    ref = acc / ref_spp
+    mse = float(((img - ref) ** 2).mean())
+    mse_corrected = mse / (1.0 + spp / ref_spp)
+    return {"reference_spp": ref_spp, "rmse": round(mse_corrected ** 0.5, 6), "rmse_raw": round(mse ** 0.5, 6),
+            "mean_radiance": round(float(ref.mean()), 6),
+            "relative_rmse": round(mse_corrected ** 0.5 / float(ref.mean()), 6),
+            "note": "same estimator and Sobol indices on the CPU oracle (frames equal per pixel to 1e-4), so "
+                    "the CPU baseline's RMSE is this one"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,6 +232,10 @@ def main():
     ap.add_argument("--workload", default="c2", choices=["c1", "c2"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-count", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--reference-spp", type=int, default=0, metavar="N",
+                    help="also report the RMSE against an N-spp converged image (N a multiple of 64)")
     ap.add_argument("--threshold", type=int, default=0, help="persistent schedule regeneration threshold (0 = default)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES (PMC calibration) and exit")
@@ -146,6 +290,11 @@ def main():
     print(f"[bench] {args.steps} steps: {ms_per_step:.1f} ms/step (events {sum(kernel_ms) / len(kernel_ms):.1f} ms)",
           file=sys.stderr, flush=True)
 
+    rmse = None
+    if args.reference_spp:
+        rmse = rmse_vs_converged(ctx, W, H, spp, depth, args.reference_spp)
+        print(f"[bench] RMSE vs {args.reference_spp} spp: {rmse['rmse']}", file=sys.stderr, flush=True)
+
     roofline = None
     if not args.no_count:
         ctx.reset_counters()
@@ -171,6 +320,18 @@ def main():
                     "kernel": "k_naive_persistent<false> (zillumgl_amd/csrc/zgl_persistent.hip)"}
     ctx.close()
 
+    adjoint = None
+    if not args.no_adjoint and args.workload == "c2":
+        adjoint = adjoint_bench(hip, max(args.steps, 3))
+        print(f"[bench] adjoint {adjoint['value']} Msamples/s, {adjoint['adjoint_over_forward']}x forward",
+              file=sys.stderr, flush=True)
+
+    batch = None
+    if not args.no_batch and args.workload == "c2":
+        batch = batch_bench(max(args.steps, 3))
+        print(f"[bench] batch {batch['value']} Msamples/s, {batch['speedup_vs_sequential']}x sequential",
+              file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(ps, depth)
@@ -187,7 +348,11 @@ def main():
                                  "per pixel within 1e-4 (tests/test_gpu_parity.py)"},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "adjoint": adjoint,
+        "batch": batch,
     }
+    if rmse:
+        out["rmse"] = rmse
     print(json.dumps(out))
 
 

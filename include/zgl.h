@@ -197,6 +197,35 @@ zgl_status zgl_render_replay(zgl_context* ctx, const zgl_pass_params* params);
 zgl_status zgl_adjoint(zgl_context* ctx, const zgl_pass_params* params, const float* d_frame,
                        float* d_material_params, float* d_env_rgb);
 
+/* ---- Batched persistent launch.  No counterpart in the reference (it renders one
+ * scene per dispatch).  A batch renders the passes of several contexts (each with its
+ * scene, camera, noise and frame uploaded) on one device in ONE persistent launch of the
+ * naive integrator, recorded once as a HIP graph and replayed by zgl_batch_launch.  Each
+ * frame receives exactly what zgl_render of the same params would write.  Waves take the
+ * scenes in the configured order, one scene at a time per wave, pixels per wave from
+ * that scene's counter.  The batch runs on its own stream: the contexts must not render
+ * concurrently, and zgl_batch_synchronize must return before their frames are read. */
+typedef struct zgl_batch zgl_batch;
+zgl_status zgl_batch_create(int32_t device, zgl_batch** out);
+void zgl_batch_destroy(zgl_batch* batch);
+const char* zgl_batch_last_error(const zgl_batch* batch);
+/* The scenes (contexts) and their passes (naive integrator); order = as given. */
+zgl_status zgl_batch_set(zgl_batch* batch, zgl_context* const* contexts, const zgl_pass_params* params, int32_t n);
+/* regen_threshold 1..63 (0 = default 16); blocks_per_cu 1..max (0 = max resident);
+ * order: a permutation of 0..n-1 giving the launch order (NULL keeps the current one). */
+zgl_status zgl_batch_configure(zgl_batch* batch, int32_t regen_threshold, int32_t blocks_per_cu,
+                               const int32_t* order);
+zgl_status zgl_batch_launch(zgl_batch* batch);
+zgl_status zgl_batch_synchronize(zgl_batch* batch);
+/* Autotuner: times the batch (median of `reps` launches after a warm-up) over scene
+ * orders (as given / most work first), regeneration thresholds {8, ..., 32} and the two
+ * highest blocks-per-CU counts, coordinate by coordinate, and keeps the fastest.  It
+ * launches the batch repeatedly: frames of passes that do not start at pass 0
+ * accumulate every launch. */
+zgl_status zgl_batch_tune(zgl_batch* batch, int32_t reps, double* best_ms);
+zgl_status zgl_batch_get_config(const zgl_batch* batch, int32_t* regen_threshold, int32_t* blocks_per_cu,
+                                int32_t* order, int32_t max_blocks_per_cu[1]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -12,11 +12,10 @@
     value to FP32 rounding;
   * through the torch.autograd.Function, the gradient of an L2 image loss against a
     target rendered at parameters perturbed by N(0, 0.1) (clamped to [0, 1]) agrees
-    with fourth-order central finite differences (h = 2^-10 ~ 1e-3) on 64 parameters:
-    at least 60 within 1e-3 relative, and every one within 1e-3 plus the differences'
-    own error estimate (the spread of the fourth-order and the second-order differences
-    at steps h and h/2), which is large only for the roughness of glossy lobes, where
-    FP32 rounding of (alpha^2 - 1) cos^2 + 1 blurs a 1e-3 step.
+    with fourth-order central finite differences on 64 random parameters (material
+    steps h = 2^-10 ~ 1e-3; env steps larger, the loss being quadratic in the texels)
+    within 1e-3 relative (roughness of glossy lobes, where FP32 cannot resolve such a
+    step, is left to the FP64 oracle comparison).
 """
 import numpy as np
 import pytest
@@ -133,39 +132,44 @@ def test_autograd_l2_gradient_matches_central_differences(ctx, c2s):
         img = rd.image(pm, pe).astype(np.float64)
         return ((img - target.numpy().astype(np.float64)) ** 2).sum()
 
-    # 64 parameters: every material parameter with a gradient, then the largest env texels.
-    # Fourth-order central differences (Richardson on steps h and 2h, h = 2^-10 ~ 1e-3 of
-    # the parameter's scale): the O(h^2) term of the plain central difference is ~1e-3
-    # relative for the roughness of glossy lobes (D ~ roughness^-4 near the peak).
     def fd(kind, i):
-        """Fourth-order central difference (steps h, 2h; h = 2^-10 of the parameter's
-        scale) and an estimate of its error: its distance to the second-order ones at
-        steps h and h/2."""
+        """Fourth-order central difference, steps h and 2h.  Materials: h = 2^-10 of the
+        scale.  Env texels: the image is linear in them and the loss quadratic, so the
+        difference is exact for any step; h = 1/4 of the scale keeps the change of every
+        pixel far above FP32 resolution."""
         base = (p0 if kind == "m" else e0).ravel()
-        h = np.float32(2.0 ** -10 * max(1.0, abs(float(base[i]))))
+        h = np.float32((2.0 ** -10 if kind == "m" else 0.25) * max(1.0, abs(float(base[i]))))
         vals = {}
-        for m in (-2, -1, -0.5, 0.5, 1, 2):
+        for m in (-2, -1, 1, 2):
             q = base.copy()
             q[i] = np.float32(base[i] + np.float32(m) * h)
             q = q.reshape(p0.shape if kind == "m" else e0.shape)
             vals[m] = (loss_at(q, e0) if kind == "m" else loss_at(p0, q), float(q.ravel()[i]) - float(base[i]))
-        c = {m: (vals[m][0] - vals[-m][0]) / (vals[m][1] - vals[-m][1]) for m in (0.5, 1, 2)}
-        f4 = (4 * c[1] - c[2]) / 3
-        return f4, abs(f4 - c[1]) + abs(c[1] - c[0.5])
+        c = {m: (vals[m][0] - vals[-m][0]) / (vals[m][1] - vals[-m][1]) for m in (1, 2)}
+        return (4 * c[1] - c[2]) / 3
 
+    # 64 random parameters with a gradient.  Excluded: the roughness of materials whose
+    # GGX term (alpha^2 - 1) cos^2 + 1 cannot resolve a 1e-3 step in FP32 to 1e-4, i.e.
+    # 2 h dalpha^2/dr < 2^-23 / 1e-4 (alpha^2 = r'^4, r' = mix(0.0134, 1, r)): there the
+    # FP32 estimator is a staircase at the step's scale, and the adjoint is checked
+    # against the oracle's FP64 adjoint instead (test_adjoint_matches_oracle_fp64).
+    rough = 0.0134 * (1 - p0[:, 3].astype(np.float64)) + p0[:, 3]
+    resolvable = 2 * 2.0 ** -10 * 4 * rough ** 3 * 0.9866 >= 2.0 ** -23 / 1e-4
+    # gradients below 1e-3 of the largest of their kind are numerically zero for a
+    # difference of FP32 images and are not sampled
+    cand_m = [i for i in range(gm.size)
+              if abs(gm.ravel()[i]) >= 1e-3 * np.abs(gm).max() and (i % 5 != 3 or resolvable[i // 5])]
+    cand_e = list(np.nonzero(np.abs(ge.ravel()) >= 1e-3 * np.abs(ge).max())[0])
     rng = np.random.default_rng(9)
-    mat_idx = [i for i in np.argsort(-np.abs(gm).ravel()) if abs(gm.ravel()[i]) > 1e-3 * np.abs(gm).max()][:24]
-    env_idx = list(np.argsort(-np.abs(ge).ravel())[:200])
-    env_idx = list(rng.choice(env_idx, 64 - len(mat_idx), replace=False))
-    strict, worst = 0, 0.0
-    for kind, i in [("m", i) for i in mat_idx] + [("e", i) for i in env_idx]:
-        f4, unc = fd(kind, i)
+    picks = rng.choice(len(cand_m) + len(cand_e), 64, replace=False)
+    params = [("m", cand_m[k]) if k < len(cand_m) else ("e", cand_e[k - len(cand_m)]) for k in picks]
+    worst = 0.0
+    for kind, i in params:
+        f4 = fd(kind, i)
         g = float((gm if kind == "m" else ge).ravel()[i])
-        rel = abs(g - f4) / max(abs(f4), 1e-12)
-        strict += rel <= 1e-3
+        rel = abs(g - f4) / max(abs(f4), 1e-30)
         worst = max(worst, rel)
-        # the differences' own error: FP32 rounding of the glossy lobes' (alpha^2 - 1) is
-        # ~1e-3 of the change a step of 1e-3 in raw roughness makes there
-        assert abs(g - f4) <= 1e-3 * abs(f4) + 2 * unc, (kind, i, g, f4, unc)
-    print(f"[gpu] adjoint: 64 central differences: {strict} within 1e-3, worst rel err {worst:.3g}", flush=True)
-    assert strict >= 60
+        assert rel <= 1e-3, (kind, i, g, f4, rel)
+    n_m = sum(k == "m" for k, _ in params)
+    print(f"[gpu] adjoint: 64 central differences ({n_m} material, {64 - n_m} env), worst rel err {worst:.3g}",
+          flush=True)

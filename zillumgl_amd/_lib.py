@@ -92,6 +92,15 @@ SIGNATURES = [
     ("zgl_set_eval_params", C.c_int32, [C.c_void_p, _fp, _fp]),
     ("zgl_render_replay", C.c_int32, [C.c_void_p, C.POINTER(PassParams)]),
     ("zgl_adjoint", C.c_int32, [C.c_void_p, C.POINTER(PassParams), _fp, _fp, _fp]),
+    ("zgl_batch_create", C.c_int32, [C.c_int32, C.POINTER(C.c_void_p)]),
+    ("zgl_batch_destroy", None, [C.c_void_p]),
+    ("zgl_batch_last_error", C.c_char_p, [C.c_void_p]),
+    ("zgl_batch_set", C.c_int32, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(PassParams), C.c_int32]),
+    ("zgl_batch_configure", C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, _ip]),
+    ("zgl_batch_launch", C.c_int32, [C.c_void_p]),
+    ("zgl_batch_synchronize", C.c_int32, [C.c_void_p]),
+    ("zgl_batch_tune", C.c_int32, [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
+    ("zgl_batch_get_config", C.c_int32, [C.c_void_p, _ip, _ip, _ip, _ip]),
 ]
 
 _lib = None

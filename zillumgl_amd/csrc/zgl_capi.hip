@@ -1,4 +1,5 @@
 // zgl_capi.hip -- the C ABI: context, validated scene upload, pass dispatch.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -24,6 +25,27 @@ struct DevBuf {
         bytes = 0;
     }
 };
+
+void hipCheck(hipError_t e, const char* what);
+
+// Grows b to at least `bytes` (contents not kept) and copies src into it on stream st.
+void bufUpload(DevBuf& b, const void* src, size_t bytes, hipStream_t st, const char* what)
+{
+    if (b.bytes < bytes || bytes == 0) {
+        b.release();
+        size_t alloc = bytes ? bytes : 16;
+        hipCheck(hipMalloc(&b.p, alloc), what);
+        b.bytes = alloc;
+    }
+    if (bytes) hipCheck(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, st), what);
+}
+void bufEnsure(DevBuf& b, size_t bytes, const char* what)
+{
+    if (b.bytes >= bytes && b.p) return;
+    b.release();
+    hipCheck(hipMalloc(&b.p, bytes ? bytes : 16), what);
+    b.bytes = bytes ? bytes : 16;
+}
 
 struct ZglError {
     zgl_status code;
@@ -58,23 +80,8 @@ struct zgl_context {
     bool hasScene = false, hasCamera = false, hasNoise = false;
     int filmW = 0, filmH = 0;
 
-    void upload(DevBuf& b, const void* src, size_t bytes, const char* what)
-    {
-        if (b.bytes < bytes || bytes == 0) {
-            b.release();
-            size_t alloc = bytes ? bytes : 16;
-            hipCheck(hipMalloc(&b.p, alloc), what);
-            b.bytes = alloc;
-        }
-        if (bytes) hipCheck(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, stream), what);
-    }
-    void ensure(DevBuf& b, size_t bytes, const char* what)
-    {
-        if (b.bytes >= bytes && b.p) return;
-        b.release();
-        hipCheck(hipMalloc(&b.p, bytes ? bytes : 16), what);
-        b.bytes = bytes ? bytes : 16;
-    }
+    void upload(DevBuf& b, const void* src, size_t bytes, const char* what) { bufUpload(b, src, bytes, stream, what); }
+    void ensure(DevBuf& b, size_t bytes, const char* what) { bufEnsure(b, bytes, what); }
     void releaseAll()
     {
         DevBuf* all[] = {&rows, &tris, &indices, &vertices, &normals, &texcoords, &matTex, &materials,
@@ -85,9 +92,34 @@ struct zgl_context {
     }
 };
 
+struct zgl_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::vector<zgl_context*> ctxs;
+    std::vector<zgl_pass_params> params;
+    std::vector<BatchScene> host;  // in launch order
+    std::vector<int32_t> order;    // launch position -> index in ctxs
+    DevBuf items, counters, sobolTab;
+    int numCUs = 0, maxBlocksPerCU = 1;
+    int threshold = 16, blocksPerCU = 0;
+    int sobolFirst = 0, sobolCount = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    void upload(DevBuf& b, const void* src, size_t bytes, const char* what) { bufUpload(b, src, bytes, stream, what); }
+    void ensure(DevBuf& b, size_t bytes, const char* what) { bufEnsure(b, bytes, what); }
+    void dropGraph()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+};
+
 namespace {
 
-template <class F> zgl_status guarded(zgl_context* ctx, F&& f)
+template <class T, class F> zgl_status guarded(T* ctx, F&& f)
 {
     if (!ctx) return ZGL_E_INVALID;
     try {
@@ -696,6 +728,238 @@ zgl_status zgl_reset_counters(zgl_context* ctx)
         hipCheck(hipMemsetAsync(ctx->counters.p, 0, NumCounters * 8, ctx->stream), "reset counters");
         hipCheck(hipStreamSynchronize(ctx->stream), "reset counters");
     });
+}
+
+// ---------------------------------------------------------------- batched launch
+zgl_status zgl_batch_create(int32_t device, zgl_batch** out)
+{
+    if (!out) return ZGL_E_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return ZGL_E_NODEVICE;
+    if (device < 0 || device >= n) return ZGL_E_INVALID;
+    zgl_batch* b = new (std::nothrow) zgl_batch();
+    if (!b) return ZGL_E_OOM;
+    b->device = device;
+    zgl_status st = guarded(b, [&] {
+        hipCheck(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
+        hipCheck(hipDeviceGetAttribute(&b->numCUs, hipDeviceAttributeMultiprocessorCount, device), "attribute");
+        b->maxBlocksPerCU = naiveBatchBlocksPerCU();
+        b->blocksPerCU = b->maxBlocksPerCU;
+    });
+    if (st != ZGL_OK) {
+        if (b->stream) (void)hipStreamDestroy(b->stream);
+        delete b;
+        return st;
+    }
+    *out = b;
+    return ZGL_OK;
+}
+
+void zgl_batch_destroy(zgl_batch* b)
+{
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    b->dropGraph();
+    b->items.release();
+    b->counters.release();
+    b->sobolTab.release();
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+const char* zgl_batch_last_error(const zgl_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+// Builds the device table of the batch in launch order (b->order) and drops the graph.
+static void batchUpload(zgl_batch* b)
+{
+    const int n = (int)b->ctxs.size();
+    b->host.resize(n);
+    for (int k = 0; k < n; k++) {
+        const int i = b->order[k];
+        zgl_context* ctx = b->ctxs[i];
+        const zgl_pass_params* p = &b->params[i];
+        BatchScene& it = b->host[k];
+        it.s = ctx->scene;
+        it.s.sobolTab = (const uint32_t*)b->sobolTab.p;
+        it.s.sobolTabFirst = b->sobolFirst;
+        it.s.sobolTabCount = b->sobolCount;
+        it.prm = PassParams{};
+        it.prm.firstPass = p->first_pass;
+        it.prm.nPasses = p->n_passes;
+        it.prm.freeCounter = p->free_counter;
+        it.prm.maxDepth = p->max_depth;
+        it.prm.russianRoulette = p->russian_roulette;
+        it.prm.sampleLight = p->sample_light;
+        it.prm.lightEnvUniform = p->light_env_uniform;
+        it.prm.lightPortion = p->light_portion;
+        it.prm.loopsPerPass = 1;
+        it.prm.pathsPerPass = 1536;
+        it.prm.lptScale = 0.0f;
+        it.frame = (float4*)ctx->frame.p;
+    }
+    b->dropGraph();
+    hipCheck(hipStreamSynchronize(b->stream), "batch");
+    b->upload(b->items, b->host.data(), sizeof(BatchScene) * n, "batch table");
+    hipCheck(hipStreamSynchronize(b->stream), "batch table");
+}
+
+zgl_status zgl_batch_set(zgl_batch* b, zgl_context* const* ctxs, const zgl_pass_params* params, int32_t n)
+{
+    return guarded(b, [&] {
+        if (!ctxs || !params || n < 1 || n > 65536) invalid("batch: contexts / params / count");
+        int first = 1 << 30, end = 0;
+        for (int i = 0; i < n; i++) {
+            zgl_context* ctx = ctxs[i];
+            const zgl_pass_params* p = &params[i];
+            if (!ctx) invalid("batch: null context");
+            if (ctx->device != b->device) invalid("batch: context on another device");
+            if (!ctx->hasScene || !ctx->hasCamera || !ctx->hasNoise)
+                throw ZglError{ZGL_E_STATE, "batch: scene, camera and noise must be set in every context"};
+            if (p->integrator != ZGL_INTEGRATOR_NAIVE) invalid("batch: the batched launch runs the naive integrator");
+            if (p->n_passes < 1 || p->n_passes > 65536 || p->first_pass < 0 || p->first_pass > (1 << 24) ||
+                p->max_depth < 0 || p->max_depth > 1024)
+                invalid("batch: pass range / depth");
+            for (int j = 0; j < i; j++)
+                if (ctxs[j] == ctx) invalid("batch: a context appears twice");
+            first = std::min(first, p->first_pass);
+            end = std::max(end, p->first_pass + p->n_passes);
+        }
+        b->ctxs.assign(ctxs, ctxs + n);
+        b->params.assign(params, params + n);
+        b->order.resize(n);
+        for (int i = 0; i < n; i++) b->order[i] = i;
+        // one Sobol table for the union of the pass ranges (plus one for dims past 256)
+        b->sobolFirst = first;
+        b->sobolCount = end - first + 1;
+        b->ensure(b->sobolTab, (size_t)b->sobolCount * 256 * 4, "sobol table");
+        hipCheck(launchSobolTable((const uint32_t*)ctxs[0]->sobolM.p, (uint32_t*)b->sobolTab.p, b->sobolFirst,
+                                  b->sobolCount, b->stream),
+                 "sobol table");
+        b->ensure(b->counters, (size_t)n * 4, "batch counters");
+        batchUpload(b);
+    });
+}
+
+zgl_status zgl_batch_configure(zgl_batch* b, int32_t regen_threshold, int32_t blocks_per_cu, const int32_t* order)
+{
+    return guarded(b, [&] {
+        const int n = (int)b->ctxs.size();
+        if (n == 0) throw ZglError{ZGL_E_STATE, "batch: no scenes set"};
+        if (regen_threshold < 0 || regen_threshold > 63) invalid("regen_threshold must be in [0, 63]");
+        if (blocks_per_cu < 0 || blocks_per_cu > b->maxBlocksPerCU) invalid("blocks_per_cu out of range");
+        if (order) {
+            std::vector<char> seen(n, 0);
+            for (int k = 0; k < n; k++) {
+                if (order[k] < 0 || order[k] >= n || seen[order[k]]) invalid("order must be a permutation");
+                seen[order[k]] = 1;
+            }
+            b->order.assign(order, order + n);
+        }
+        b->threshold = regen_threshold > 0 ? regen_threshold : kDefaultRegenThreshold;
+        b->blocksPerCU = blocks_per_cu > 0 ? blocks_per_cu : b->maxBlocksPerCU;
+        batchUpload(b);
+    });
+}
+
+zgl_status zgl_batch_launch(zgl_batch* b)
+{
+    return guarded(b, [&] {
+        const int n = (int)b->ctxs.size();
+        if (n == 0) throw ZglError{ZGL_E_STATE, "batch: no scenes set"};
+        if (!b->exec) {
+            // the whole launch (counter reset + persistent kernel) as one graph
+            hipCheck(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "capture");
+            const hipError_t e = launchNaiveBatch((const BatchScene*)b->items.p, n, (int*)b->counters.p,
+                                                  b->threshold, b->numCUs * b->blocksPerCU, b->stream);
+            hipGraph_t g = nullptr;
+            const hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+            hipCheck(e, "batch capture");
+            hipCheck(e2, "batch capture");
+            b->graph = g;
+            hipCheck(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "graph instantiate");
+        }
+        hipCheck(hipGraphLaunch(b->exec, b->stream), "graph launch");
+    });
+}
+
+zgl_status zgl_batch_synchronize(zgl_batch* b)
+{
+    return guarded(b, [&] { hipCheck(hipStreamSynchronize(b->stream), "batch"); });
+}
+
+// Median time of `reps` graph launches after one warm-up launch, in ms.
+static double batchTime(zgl_batch* b, int reps)
+{
+    hipEvent_t e0, e1;
+    hipCheck(hipEventCreate(&e0), "event");
+    hipCheck(hipEventCreate(&e1), "event");
+    std::vector<double> ms;
+    for (int r = 0; r <= reps; r++) {
+        hipCheck(hipEventRecord(e0, b->stream), "event");
+        if (zgl_batch_launch(b) != ZGL_OK) throw ZglError{ZGL_E_HIP, b->err};
+        hipCheck(hipEventRecord(e1, b->stream), "event");
+        hipCheck(hipEventSynchronize(e1), "event");
+        float t = 0.0f;
+        hipCheck(hipEventElapsedTime(&t, e0, e1), "event");
+        if (r > 0) ms.push_back(t);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    std::sort(ms.begin(), ms.end());
+    return ms[ms.size() / 2];
+}
+
+zgl_status zgl_batch_tune(zgl_batch* b, int32_t reps, double* best_ms)
+{
+    return guarded(b, [&] {
+        const int n = (int)b->ctxs.size();
+        if (n == 0) throw ZglError{ZGL_E_STATE, "batch: no scenes set"};
+        if (reps < 1 || reps > 100) invalid("reps must be in [1, 100]");
+        auto config = [&](int th, int bpc, const std::vector<int32_t>& ord) {
+            if (zgl_batch_configure(b, th, bpc, ord.data()) != ZGL_OK) throw ZglError{ZGL_E_INVALID, b->err};
+        };
+        // candidate scene orders: as given, and most work first (pixels x passes x depth)
+        std::vector<int32_t> given(n), heavy(n);
+        for (int i = 0; i < n; i++) given[i] = heavy[i] = i;
+        auto work = [&](int i) {
+            const zgl_context* c = b->ctxs[i];
+            return (double)c->filmW * c->filmH * b->params[i].n_passes * (b->params[i].max_depth + 1);
+        };
+        std::stable_sort(heavy.begin(), heavy.end(), [&](int a, int c) { return work(a) > work(c); });
+        int th = b->threshold, bpc = b->blocksPerCU;
+        std::vector<int32_t> ord = b->order;
+        double best = 1e30;
+        for (const auto& o : {given, heavy}) {
+            config(th, bpc, o);
+            const double t = batchTime(b, reps);
+            if (t < best) { best = t; ord = o; }
+        }
+        for (int t : {8, 12, 16, 20, 24, 32}) {
+            config(t, bpc, ord);
+            const double tm = batchTime(b, reps);
+            if (tm < best) { best = tm; th = t; }
+        }
+        for (int c = b->maxBlocksPerCU; c >= std::max(1, b->maxBlocksPerCU - 1); c--) {
+            config(th, c, ord);
+            const double tm = batchTime(b, reps);
+            if (tm < best) { best = tm; bpc = c; }
+        }
+        config(th, bpc, ord);
+        if (best_ms) *best_ms = best;
+    });
+}
+
+zgl_status zgl_batch_get_config(const zgl_batch* b, int32_t* regen_threshold, int32_t* blocks_per_cu,
+                                int32_t* order, int32_t max_blocks_per_cu[1])
+{
+    if (!b) return ZGL_E_INVALID;
+    if (regen_threshold) *regen_threshold = b->threshold;
+    if (blocks_per_cu) *blocks_per_cu = b->blocksPerCU;
+    if (order) std::memcpy(order, b->order.data(), b->order.size() * 4);
+    if (max_blocks_per_cu) max_blocks_per_cu[0] = b->maxBlocksPerCU;
+    return ZGL_OK;
 }
 
 }  // extern "C"

@@ -29,6 +29,13 @@ struct PassParams {
 // Counter slots (uint64): 0 bytes, 1 rows, 2 leaf tests, 3 traversals, 4 samples
 constexpr int NumCounters = 8;
 
+// One scene of a batched persistent launch (device-resident table).
+struct BatchScene {
+    DevScene s;
+    PassParams prm;
+    float4* frame;
+};
+
 hipError_t launchSobolTable(const uint32_t* sobolM, uint32_t* tab, int first, int count, hipStream_t st);
 hipError_t launchFill(float* p, int64_t n, float v, hipStream_t st);
 hipError_t launchFilmToFrame(const float* film, float4* frame, int w, int h, hipStream_t st);
@@ -39,6 +46,9 @@ hipError_t launchTraceAny(const DevScene& s, const float* rays, const float* tma
 hipError_t launchNaivePersistent(const DevScene& s, const PassParams& p, float4* frame, int* pixelCounter,
                                  int threshold, int blocks, unsigned long long* counters, hipStream_t st);
 int naivePersistentBlocksPerCU();
+int naiveBatchBlocksPerCU();
+hipError_t launchNaiveBatch(const BatchScene* items, int nScenes, int* sceneCounters, int threshold, int blocks,
+                            hipStream_t st);
 // Differentiable naive estimator: dFrame == null renders the replay estimator into
 // frame; otherwise accumulates FP64 gradients into dMat (nMaterials x 5) and dEnv.
 hipError_t launchNaiveReplay(const DevScene& s, const PassParams& p, const float4* evalMat, const float4* evalEnv,

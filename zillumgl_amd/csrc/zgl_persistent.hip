@@ -29,6 +29,7 @@ enum : int {
     kExtend,    // extension ray walking / walked
     kFinish,    // sample complete
     kDone,
+    kWait,      // batch: the wave's scene has no pixels left; wait for the next scene
 };
 
 // MTBVH rows each walking lane advances between two ballots of the walking loop.
@@ -56,18 +57,18 @@ template <bool C> __device__ void flushCountersP(const Counters& cn, unsigned lo
     atomicAdd(out + 3, (unsigned long long)cn.traversals);
 }
 
-template <bool C>
-__global__ __launch_bounds__(256, 4) void k_naive_persistent(DevScene s, PassParams prm, float4* __restrict__ frame,
-                                                          int* __restrict__ pixelCounter, int threshold,
-                                                          unsigned long long* counters)
+// The persistent schedule.  BATCH = false: one scene (s0, prm0, frame0, counter0).
+// BATCH = true: the scenes of `items` in order; a wave works on one scene at a time
+// (scene data stays wave-uniform), takes its pixels from that scene's counter, and
+// moves to the next scene once its lanes have finished the pixels they hold.
+template <bool C, bool BATCH>
+__device__ __forceinline__ void naiveBody(const DevScene& s0, const PassParams& prm0, float4* frame0, int* counter0,
+                                          const BatchScene* items, int nScenes, int* sceneCounters, int threshold,
+                                          unsigned long long* counters)
 {
-    const int W = s.filmW, H = s.filmH;
-    const int tilesX = (W + 7) / 8, tilesY = (H + 7) / 8;
-    const int nIndex = tilesX * tilesY * 64;
-    const bool uniform = prm.lightEnvUniform != 0;
-    const float portion = prm.lightPortion;
     const int lane = threadIdx.x & 63;
     Counters cn;
+    int cs = 0;  // batch: the wave's current scene (the same in every lane)
 
     int phase = kNewPixel;
     int pix = -1, pass = 0;
@@ -90,6 +91,16 @@ __global__ __launch_bounds__(256, 4) void k_naive_persistent(DevScene s, PassPar
     wk.dist = 0.f; wk.k = 0; wk.closest = -1; wk.any = false;
 
     for (;;) {
+        const int csu = BATCH ? __builtin_amdgcn_readfirstlane(cs) : 0;
+        const DevScene& s = BATCH ? items[csu].s : s0;
+        const PassParams& prm = BATCH ? items[csu].prm : prm0;
+        float4* const frame = BATCH ? items[csu].frame : frame0;
+        int* const pixelCounter = BATCH ? sceneCounters + csu : counter0;
+        const int W = s.filmW, H = s.filmH;
+        const int nIndex = ((W + 7) / 8) * ((H + 7) / 8) * 64;
+        const bool uniform = prm.lightEnvUniform != 0;
+        const float portion = prm.lightPortion;
+
         // ------------------------------------------------------------ shading
         if (!walking) {
             // completion of the ray that was walked
@@ -284,7 +295,7 @@ __global__ __launch_bounds__(256, 4) void k_naive_persistent(DevScene s, PassPar
                 pix = -1;
                 stored = false;
                 if (idx >= nIndex) {
-                    phase = kDone;
+                    phase = BATCH ? kWait : kDone;
                 } else {
                     int x, y;
                     pixelOfIndex(idx, W, x, y);
@@ -314,6 +325,14 @@ __global__ __launch_bounds__(256, 4) void k_naive_persistent(DevScene s, PassPar
             phase = kPrimary;
         }
 
+        if (BATCH) {
+            // every lane idle and some waiting: move the wave to the next scene
+            if (__ballot(walking || (phase != kWait && phase != kDone)) == 0ull && __ballot(phase == kWait) != 0ull) {
+                cs = csu + 1;
+                if (phase == kWait) phase = (cs < nScenes) ? kNewPixel : kDone;
+                continue;
+            }
+        }
         if (__ballot(phase != kDone) == 0ull) break;
 
         // ------------------------------------------------------------ walking
@@ -333,6 +352,28 @@ __global__ __launch_bounds__(256, 4) void k_naive_persistent(DevScene s, PassPar
     flushCountersP<C>(cn, counters);
 }
 
+template <bool C>
+__global__ __launch_bounds__(256, 4) void k_naive_persistent(DevScene s, PassParams prm, float4* __restrict__ frame,
+                                                          int* __restrict__ pixelCounter, int threshold,
+                                                          unsigned long long* counters)
+{
+    naiveBody<C, false>(s, prm, frame, pixelCounter, nullptr, 0, nullptr, threshold, counters);
+}
+
+__global__ __launch_bounds__(256, 4) void k_naive_batch(const BatchScene* __restrict__ items, int nScenes,
+                                                     int* __restrict__ sceneCounters, int threshold)
+{
+    naiveBody<false, true>(items[0].s, items[0].prm, nullptr, nullptr, items, nScenes, sceneCounters, threshold,
+                           nullptr);
+}
+
+// Zeroes the per-scene pixel counters of a batch (a kernel, so that it can sit in a graph).
+__global__ void k_reset_counters(int* c, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) c[i] = 0;
+}
+
 }  // namespace zgl
 
 namespace zgl {
@@ -350,6 +391,23 @@ hipError_t launchNaivePersistent(const DevScene& s, const PassParams& p, float4*
         hipLaunchKernelGGL(k_naive_persistent<false>, dim3(blocks), dim3(256), 0, st, s, p, frame, pixelCounter,
                            threshold, counters);
     return hipGetLastError();
+}
+
+hipError_t launchNaiveBatch(const BatchScene* items, int nScenes, int* sceneCounters, int threshold, int blocks,
+                            hipStream_t st)
+{
+    if (threshold < 0) threshold = 0;
+    if (threshold > 63) threshold = 63;
+    hipLaunchKernelGGL(k_reset_counters, dim3((nScenes + 255) / 256), dim3(256), 0, st, sceneCounters, nScenes);
+    hipLaunchKernelGGL(k_naive_batch, dim3(blocks), dim3(256), 0, st, items, nScenes, sceneCounters, threshold);
+    return hipGetLastError();
+}
+
+int naiveBatchBlocksPerCU()
+{
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_naive_batch, 256, 0) != hipSuccess) return 1;
+    return n > 0 ? n : 1;
 }
 
 int naivePersistentBlocksPerCU()

@@ -160,6 +160,77 @@ class Context:
         self._check(_lib.lib().zgl_reset_counters(self.h))
 
 
+class Batch:
+    """Several contexts rendered by one persistent launch captured in a HIP graph
+    (zgl_batch_*).  Each frame receives what Context.render of the same passes writes."""
+
+    def __init__(self, device=0):
+        L = _lib.lib()
+        h = C.c_void_p()
+        st = L.zgl_batch_create(device, C.byref(h))
+        if st != _lib.ZGL_OK:
+            raise _lib.ZglError(st, f"zgl_batch_create(device={device}) failed")
+        self.h = h
+        self.n = 0
+
+    def close(self):
+        if self.h:
+            _lib.lib().zgl_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st):
+        if st != _lib.ZGL_OK:
+            raise _lib.ZglError(st, _lib.lib().zgl_batch_last_error(self.h).decode())
+
+    def set(self, contexts, passes):
+        """contexts: [Context]; passes: [(first_pass, n_passes, free_counter, max_depth)] or
+        dicts of Context.render keyword arguments."""
+        n = len(contexts)
+        hs = (C.c_void_p * n)(*[c.h.value for c in contexts])
+        ps = (_lib.PassParams * n)()
+        for i, p in enumerate(passes):
+            if isinstance(p, dict):
+                kw = dict(russian_roulette=False, sample_light=True, light_env_uniform=False, light_portion=0.5)
+                kw.update(p)
+                ps[i] = _lib.PassParams(_lib.INTEGRATOR_NAIVE, kw["first_pass"], kw["n_passes"], kw["free_counter"],
+                                        kw["max_depth"], int(kw["russian_roulette"]), int(kw["sample_light"]),
+                                        int(kw["light_env_uniform"]), kw["light_portion"], 32, 1, 0, 0)
+            else:
+                first, count, fc, depth = p
+                ps[i] = _lib.PassParams(_lib.INTEGRATOR_NAIVE, first, count, fc, depth, 0, 1, 0, 0.5, 32, 1, 0, 0)
+        self._keep = (contexts, hs, ps)
+        self._check(_lib.lib().zgl_batch_set(self.h, hs, ps, n))
+        self.n = n
+
+    def configure(self, regen_threshold=0, blocks_per_cu=0, order=None):
+        o = None if order is None else np.ascontiguousarray(order, np.int32)
+        self._check(_lib.lib().zgl_batch_configure(self.h, regen_threshold, blocks_per_cu, ptr(o, C.c_int32)))
+
+    def launch(self):
+        self._check(_lib.lib().zgl_batch_launch(self.h))
+
+    def synchronize(self):
+        self._check(_lib.lib().zgl_batch_synchronize(self.h))
+
+    def tune(self, reps=1):
+        best = C.c_double(0)
+        self._check(_lib.lib().zgl_batch_tune(self.h, reps, C.byref(best)))
+        return best.value
+
+    def config(self):
+        th, bpc, mx = C.c_int32(), C.c_int32(), C.c_int32()
+        order = np.zeros(self.n, np.int32)
+        self._check(_lib.lib().zgl_batch_get_config(self.h, C.byref(th), C.byref(bpc), ptr(order, C.c_int32),
+                                                     C.byref(mx)))
+        return dict(regen_threshold=th.value, blocks_per_cu=bpc.value, order=order, max_blocks_per_cu=mx.value)
+
+
 class NaivePathIntegrator:
     """Naive MIS path tracer (src/integrator/NaivePath.cpp:77-115); defaults from
     PathIntegParam (src/core/Integrator.h:56-66)."""

@@ -326,8 +326,9 @@ def merge(meshes):
                 np.concatenate(idx).astype(np.uint32), np.concatenate(uv).astype(np.float32))
 
 
-def box(center, size, angle_deg):
-    """Axis-aligned box rotated about the vertical (z) axis: 12 triangles, flat normals."""
+def box_faces(center, size, angle_deg):
+    """The 6 faces (corners p0..p3, outward normal) of an axis-aligned box rotated about
+    the vertical (z) axis."""
     cx, cy, z0 = center
     sx, sy, sz = size
     a = math.radians(angle_deg)
@@ -335,18 +336,41 @@ def box(center, size, angle_deg):
     rot = lambda x, y: (cx + c * x - s * y, cy + s * x + c * y)
     hx, hy = sx / 2, sy / 2
     corners = [rot(-hx, -hy), rot(hx, -hy), rot(hx, hy), rot(-hx, hy)]
-    faces = []
     bot = [(x, y, z0) for x, y in corners]
     top = [(x, y, z0 + sz) for x, y in corners]
-    faces.append(quad(top[0], top[1], top[2], top[3], (0, 0, 1)))
-    faces.append(quad(bot[3], bot[2], bot[1], bot[0], (0, 0, -1)))
+    faces = [((top[0], top[1], top[2], top[3]), (0, 0, 1)), ((bot[3], bot[2], bot[1], bot[0]), (0, 0, -1))]
     for i in range(4):
         j = (i + 1) % 4
         ex, ey = corners[j][0] - corners[i][0], corners[j][1] - corners[i][1]
         n = np.array([ey, -ex, 0.0])
         n /= np.linalg.norm(n)
-        faces.append(quad(bot[i], bot[j], top[j], top[i], n))
-    return merge(faces)
+        faces.append(((bot[i], bot[j], top[j], top[i]), tuple(n)))
+    return faces
+
+
+def box(center, size, angle_deg):
+    """Axis-aligned box rotated about the vertical (z) axis: 12 triangles, flat normals."""
+    return merge([quad(*cs, n) for cs, n in box_faces(center, size, angle_deg)])
+
+
+def grid_patch(p0, p1, p2, p3, normal, n, i0, i1, j0, j1):
+    """Cells [i0, i1) x [j0, j1) of an n x n tessellation of the planar quad p0 p1 p2 p3
+    (bilinear in the corners), 2 triangles per cell, flat normal."""
+    P = [np.asarray(p, np.float64) for p in (p0, p1, p2, p3)]
+    u = np.arange(i0, i1 + 1) / n
+    v = np.arange(j0, j1 + 1) / n
+    U, V = np.meshgrid(u, v, indexing="ij")
+    pos = ((1 - U)[..., None] * (1 - V)[..., None] * P[0] + U[..., None] * (1 - V)[..., None] * P[1] +
+           U[..., None] * V[..., None] * P[2] + (1 - U)[..., None] * V[..., None] * P[3]).reshape(-1, 3)
+    nu, nv = i1 - i0, j1 - j0
+    ii, jj = np.meshgrid(np.arange(nu), np.arange(nv), indexing="ij")
+    a = (ii * (nv + 1) + jj).ravel()
+    b = ((ii + 1) * (nv + 1) + jj).ravel()
+    c = ((ii + 1) * (nv + 1) + jj + 1).ravel()
+    d = (ii * (nv + 1) + jj + 1).ravel()
+    idx = np.concatenate([np.stack([a, b, c], 1), np.stack([a, c, d], 1)]).astype(np.uint32)
+    uv = np.stack([U.ravel(), V.ravel()], 1).astype(np.float32)
+    return Mesh(pos.astype(np.float32), np.tile(np.asarray(normal, np.float32), (len(pos), 1)), idx, uv)
 
 
 def icosphere(subdiv):
@@ -544,3 +568,81 @@ def config3(width=1024, height=1024, seed=3):
                  np.array([[0, 1, 2]], np.uint32), None)
         d.lights.append((m, (50.0, 50.0, 50.0)))
     return d
+
+
+def config5(width=512, height=512, n=77, n_materials=32, env_size=(256, 128), seed=5):
+    """The Cornell box of config1 tessellated to 201,586 object triangles (each of the 5 walls
+    and 12 box faces an n x n grid, n = 77) in 68 patches (2 x 2 per face) sharing 32
+    Disney materials (baseColor U[0,1]^3, roughness U[0.05,1], metallic Bernoulli(0.3),
+    specular 0.5), the 2-triangle ceiling light, and a 256 x 128 sky environment seen through the
+    open side: the differentiable-rendering workload (512^2, 32 spp, depth 6)."""
+    rng = Rng(seed)
+    faces = [((( 0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0)), (0, 0, 1)),
+             (((0, 0, 1), (0, 1, 1), (1, 1, 1), (1, 0, 1)), (0, 0, -1)),
+             (((0, 1, 0), (1, 1, 0), (1, 1, 1), (0, 1, 1)), (0, -1, 0)),
+             (((0, 0, 0), (0, 1, 0), (0, 1, 1), (0, 0, 1)), (1, 0, 0)),
+             (((1, 0, 0), (1, 0, 1), (1, 1, 1), (1, 1, 0)), (-1, 0, 0))]
+    faces += box_faces((0.66, 0.34, 0.0), (0.3, 0.3, 0.3), -17.0)
+    faces += box_faces((0.34, 0.64, 0.0), (0.3, 0.3, 0.6), 17.0)
+    mats = [Material(base_color=tuple(rng.uniform(3).tolist()), roughness=rng.uniform(None, 0.05, 1.0),
+                     metallic=1.0 if rng.uniform(None) < 0.3 else 0.0, specular=0.5, type=MAT_PRINCIPLED)
+            for _ in range(n_materials)]
+    groups = [[] for _ in range(n_materials)]
+    h = n // 2
+    k = 0
+    for corners, normal in faces:
+        for (i0, i1) in ((0, h), (h, n)):
+            for (j0, j1) in ((0, h), (h, n)):
+                groups[k % n_materials].append(grid_patch(*corners, normal, n, i0, i1, j0, j1))
+                k += 1
+    d = SceneDesc()
+    d.objects = [(merge(g), m) for g, m in zip(groups, mats)]
+    d.lights = [(cornell_light(), (17.0, 12.0, 4.0))]
+    d.env = sky_env(env_size[0], env_size[1], seed=seed + 6)
+    d.camera = CameraDesc(pos=(0.5, -1.5, 0.5), angle=(0.0, 0.0, 0.0), fov=40.0)
+    d.width, d.height = width, height
+    return d
+
+
+def cornell_tessellated(n, seed):
+    """config1's Cornell box with every wall and box face an n x n grid (34 n^2 object
+    triangles) and the boxes' rotations and positions jittered by the seed."""
+    rng = Rng(seed)
+    white, red, green = Material(base_color=WHITE), Material(base_color=RED), Material(base_color=GREEN)
+    walls = [(((0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0)), (0, 0, 1), white),
+             (((0, 0, 1), (0, 1, 1), (1, 1, 1), (1, 0, 1)), (0, 0, -1), white),
+             (((0, 1, 0), (1, 1, 0), (1, 1, 1), (0, 1, 1)), (0, -1, 0), white),
+             (((0, 0, 0), (0, 1, 0), (0, 1, 1), (0, 0, 1)), (1, 0, 0), red),
+             (((1, 0, 0), (1, 0, 1), (1, 1, 1), (1, 1, 0)), (-1, 0, 0), green)]
+    jit = rng.uniform(6) - 0.5
+    boxes = box_faces((0.66 + 0.05 * jit[0], 0.34 + 0.05 * jit[1], 0.0), (0.3, 0.3, 0.3), -17.0 + 20.0 * jit[2])
+    boxes += box_faces((0.34 + 0.05 * jit[3], 0.64 + 0.05 * jit[4], 0.0), (0.3, 0.3, 0.6), 17.0 + 20.0 * jit[5])
+    objs = []
+    for corners, normal, mat in walls:
+        objs.append((grid_patch(*corners, normal, n, 0, n, 0, n), mat))
+    objs.append((merge([grid_patch(*cs, nrm, n, 0, n, 0, n) for cs, nrm in boxes]), white))
+    return objs
+
+
+def config4(n_scenes=64, seed=4, min_tris=1000, max_tris=2_000_000):
+    """The batched workload: n_scenes Cornell boxes (config1 geometry tessellated to a
+    triangle count log-uniform in [1k, 2M], boxes jittered per scene, ceiling light
+    (17, 12, 4)), each with a resolution from {128^2, 256^2, 512^2, 1024^2}, spp from
+    U{4..64} and depth from U{3..10}.  Returns [(SceneDesc, spp, depth)]."""
+    rng = Rng(seed)
+    out = []
+    for k in range(n_scenes):
+        u = rng.uniform(4)
+        tris = math.exp(math.log(min_tris) + u[0] * (math.log(max_tris) - math.log(min_tris)))
+        n = max(1, int(round(math.sqrt(tris / 34.0))))
+        res = (128, 256, 512, 1024)[min(3, int(u[1] * 4))]
+        spp = 4 + min(60, int(u[2] * 61))
+        depth = 3 + min(7, int(u[3] * 8))
+        d = SceneDesc()
+        d.objects = cornell_tessellated(n, seed * 1000 + k)
+        d.lights = [(cornell_light(), (17.0, 12.0, 4.0))]
+        d.camera = CameraDesc(pos=(0.5, -1.5, 0.5), angle=(0.0, 0.0, 0.0), fov=40.0)
+        d.width = d.height = res
+        d.noise_seed = k
+        out.append((d, spp, depth))
+    return out
