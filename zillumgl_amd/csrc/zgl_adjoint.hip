@@ -27,8 +27,9 @@
 //   * for an environment hit: delta . T * w * dLe/dtexel,
 // where delta is dLoss/dframe at the pixel and df/dtheta comes from the dual-number
 // BSDF (zgl_dual.h).  Nothing per bounce is stored.  Gradients are accumulated with
-// FP64 atomics (per block in LDS for the material parameters): contributions of both
-// signs from many samples cancel, which FP32 sums would not resolve.
+// FP64 atomics (per block in LDS for the material parameters, into interleaved copies
+// for the texels): contributions of both signs from many samples cancel, which FP32
+// sums would not resolve.
 #include "zgl_dual.h"
 #include "zgl_internal.h"
 
@@ -72,10 +73,14 @@ ZGL_D vec3 envLeEval(const DevScene& s, const float4* env, vec3 wi, EnvTaps& tap
 // is known to be LDS and compiles to LDS atomics.
 __shared__ double sMatGrad[kLdsMaterials * NumMatGrad];
 
+// Environment gradients go to one of several copies of the gradient array, picked by
+// the block: next-event estimation concentrates on the brightest texels (a sun), and
+// atomics from the whole device on the same few addresses serialise.  The copies are
+// summed afterwards (k_sum_copies).
 struct Grads {
-    bool lds;     // material gradients go to sMatGrad
-    double* mat;  // nMaterials * NumMatGrad (global)
-    double* env;  // envW * envH * 3 (global)
+    bool lds;      // material gradients go to sMatGrad
+    double* mat;   // nMaterials * NumMatGrad (global)
+    double* env;   // this block's copy of the envW * envH * 3 gradients
 };
 ZGL_D void addEnvGrad(const Grads& g, const EnvTaps& tap, vec3 wgt)
 {
@@ -293,14 +298,15 @@ ZGL_D vec3 replayPath(const DevScene& s, const Replay& rp, const PassParams& prm
 template <int MODE>
 __global__ __launch_bounds__(256) void k_naive_replay(DevScene s, PassParams prm, Replay rp, float4* __restrict__ frame,
                                                       const float4* __restrict__ dFrame, double* __restrict__ dMat,
-                                                      double* __restrict__ dEnv)
+                                                      double* __restrict__ dEnv, int envCopies)
 {
     const bool useLds = MODE == 1 && s.nMaterials <= kLdsMaterials;
     if (useLds) {
         for (int i = threadIdx.x; i < kLdsMaterials * NumMatGrad; i += blockDim.x) sMatGrad[i] = 0.0;
         __syncthreads();
     }
-    const Grads gr{useLds, dMat, dEnv};
+    const int copy = (int)((blockIdx.y * gridDim.x + blockIdx.x) % (unsigned)envCopies);
+    const Grads gr{useLds, dMat, dEnv + (size_t)copy * ((size_t)s.envW * s.envH * 3)};
     const int x = blockIdx.x * 16 + (threadIdx.x & 15);
     const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
     if (x < s.filmW && y < s.filmH) {
@@ -364,15 +370,31 @@ hipError_t launchEnvExpand(const ushort4* env, float4* out, int64_t n, hipStream
     return hipGetLastError();
 }
 
+// dst[i] = sum over c of src[c * n + i], in FP64.
+__global__ void k_sum_copies(const double* __restrict__ src, double* __restrict__ dst, int64_t n, int copies)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double v = 0.0;
+    for (int c = 0; c < copies; c++) v += src[(size_t)c * n + i];
+    dst[i] = v;
+}
+
 hipError_t launchNaiveReplay(const DevScene& s, const PassParams& p, const float4* evalMat, const float4* evalEnv,
-                             float4* frame, const float4* dFrame, double* dMat, double* dEnv, hipStream_t st)
+                             float4* frame, const float4* dFrame, double* dMat, double* dEnv, int envCopies,
+                             double* envSum, hipStream_t st)
 {
     const Replay rp{evalMat, evalEnv};
     const dim3 grid((s.filmW + 15) / 16, (s.filmH + 15) / 16);
-    if (dFrame)
-        hipLaunchKernelGGL(k_naive_replay<1>, grid, dim3(256), 0, st, s, p, rp, frame, dFrame, dMat, dEnv);
-    else
-        hipLaunchKernelGGL(k_naive_replay<0>, grid, dim3(256), 0, st, s, p, rp, frame, dFrame, dMat, dEnv);
+    if (!dFrame) {
+        hipLaunchKernelGGL(k_naive_replay<0>, grid, dim3(256), 0, st, s, p, rp, frame, dFrame, dMat, dEnv, 1);
+        return hipGetLastError();
+    }
+    if (envCopies < 1) envCopies = 1;
+    hipLaunchKernelGGL(k_naive_replay<1>, grid, dim3(256), 0, st, s, p, rp, frame, dFrame, dMat, dEnv, envCopies);
+    const int64_t n = (int64_t)s.envW * s.envH * 3;
+    hipLaunchKernelGGL(k_sum_copies, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, dEnv, envSum, n,
+                       envCopies);
     return hipGetLastError();
 }
 

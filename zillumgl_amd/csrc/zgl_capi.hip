@@ -633,7 +633,7 @@ zgl_status zgl_render_replay(zgl_context* ctx, const zgl_pass_params* p)
         preparePasses(ctx, p, s, pp);
         if (p->integrator != ZGL_INTEGRATOR_NAIVE) invalid("the replay estimator is the naive integrator's");
         hipCheck(launchNaiveReplay(s, pp, (const float4*)ctx->evalMat.p, (const float4*)ctx->evalEnv.p,
-                                   (float4*)ctx->frame.p, nullptr, nullptr, nullptr, ctx->stream),
+                                   (float4*)ctx->frame.p, nullptr, nullptr, nullptr, 1, nullptr, ctx->stream),
                  "replay pass");
     });
 }
@@ -650,18 +650,23 @@ zgl_status zgl_adjoint(zgl_context* ctx, const zgl_pass_params* p, const float* 
         const size_t px = (size_t)ctx->filmW * ctx->filmH;
         const size_t nm = (size_t)s.nMaterials * 5, ne = (size_t)s.envW * s.envH * 3;
         ctx->upload(ctx->dFrame, d_frame, px * 16, "adjoint input");
-        // FP64 accumulators, zeroed as float pairs (all-zero bits are 0.0 in both)
+        // FP64 accumulators, zeroed as float pairs (all-zero bits are 0.0 in both).  The
+        // texel gradients get up to 16 interleaved copies (at most 256 MB), summed after.
+        int copies = 16;
+        while (copies > 1 && (size_t)copies * ne * 8 > ((size_t)256 << 20)) copies >>= 1;
         ctx->ensure(ctx->dMat, nm * 8, "material gradients");
-        ctx->ensure(ctx->dEnv, ne * 8, "environment gradients");
+        ctx->ensure(ctx->dEnv, (size_t)(copies + 1) * ne * 8, "environment gradients");
+        double* envCopies = (double*)ctx->dEnv.p;
+        double* envSum = envCopies + (size_t)copies * ne;
         hipCheck(launchFill((float*)ctx->dMat.p, (int64_t)nm * 2, 0.0f, ctx->stream), "clear");
-        hipCheck(launchFill((float*)ctx->dEnv.p, (int64_t)ne * 2, 0.0f, ctx->stream), "clear");
+        hipCheck(launchFill((float*)envCopies, (int64_t)ne * 2 * copies, 0.0f, ctx->stream), "clear");
         hipCheck(launchNaiveReplay(s, pp, (const float4*)ctx->evalMat.p, (const float4*)ctx->evalEnv.p,
                                    (float4*)ctx->frame.p, (const float4*)ctx->dFrame.p, (double*)ctx->dMat.p,
-                                   (double*)ctx->dEnv.p, ctx->stream),
+                                   envCopies, copies, envSum, ctx->stream),
                  "adjoint pass");
         std::vector<double> gm(nm), ge(ne);
         hipCheck(hipMemcpyAsync(gm.data(), ctx->dMat.p, nm * 8, hipMemcpyDeviceToHost, ctx->stream), "copy");
-        hipCheck(hipMemcpyAsync(ge.data(), ctx->dEnv.p, ne * 8, hipMemcpyDeviceToHost, ctx->stream), "copy");
+        hipCheck(hipMemcpyAsync(ge.data(), envSum, ne * 8, hipMemcpyDeviceToHost, ctx->stream), "copy");
         hipCheck(hipStreamSynchronize(ctx->stream), "adjoint");
         for (size_t i = 0; i < nm; i++) d_material_params[i] = (float)gm[i];
         for (size_t i = 0; i < ne; i++) d_env_rgb[i] = (float)ge[i];

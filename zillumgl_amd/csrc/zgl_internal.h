@@ -50,9 +50,11 @@ int naiveBatchBlocksPerCU();
 hipError_t launchNaiveBatch(const BatchScene* items, int nScenes, int* sceneCounters, int threshold, int blocks,
                             hipStream_t st);
 // Differentiable naive estimator: dFrame == null renders the replay estimator into
-// frame; otherwise accumulates FP64 gradients into dMat (nMaterials x 5) and dEnv.
+// frame; otherwise accumulates FP64 gradients into dMat (nMaterials x 5) and the envCopies
+// copies of dEnv, summed into envSum (envW x envH x 3).
 hipError_t launchNaiveReplay(const DevScene& s, const PassParams& p, const float4* evalMat, const float4* evalEnv,
-                             float4* frame, const float4* dFrame, double* dMat, double* dEnv, hipStream_t st);
+                             float4* frame, const float4* dFrame, double* dMat, double* dEnv, int envCopies,
+                             double* envSum, hipStream_t st);
 hipError_t launchEnvExpand(const ushort4* env, float4* out, int64_t n, hipStream_t st);
 hipError_t launchLight(const DevScene& s, const PassParams& p, float* film, unsigned long long* counters,
                        hipStream_t st);
