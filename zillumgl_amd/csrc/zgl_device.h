@@ -167,16 +167,18 @@ ZGL_D int traverse(const DevScene& s, const RayPre& p, float& dist, Counters& cn
     return closest;
 }
 
-// Resumable traversal: the state of one bvhHit / bvhTest walk, advanced by walkStep
-// one row (plus that row's triangle test) at a time.  The rows are visited in the same
-// order with the same tests as traverse(), so the result is the same bit for bit
-// however walks of different lanes are interleaved.
+// Resumable traversal: the state of one bvhHit / bvhTest walk.  It advances by rows
+// (walkRow); a row that is a leaf whose box is hit stops the walk there until walkLeaf
+// tests its triangle.  The rows are visited in the same order with the same tests as
+// traverse(), so the result is the same bit for bit however walks of different lanes
+// are interleaved.
 struct Walk {
     RayPre p;
     float dist;
     int k;     // absolute row index
     int end;   // one past the table's last row
     int closest;
+    int leaf;  // primitive of the leaf the walk is stopped at, -1 while it moves
     bool any;
 };
 template <bool C>
@@ -188,12 +190,13 @@ ZGL_D void walkStart(const DevScene& s, Walk& w, const Ray& r, bool any, float t
     w.k = w.p.table * s.bvhSize;
     w.end = w.k + s.bvhSize;
     w.closest = -1;
+    w.leaf = -1;
     w.any = any;
 }
-// One row (and its triangle when the row is a leaf whose box is hit); returns true
-// when the walk has finished.  `general` is true when the caller knows that no lane
-// walking with it has a special ray.
-template <bool C> ZGL_D bool walkStep(const DevScene& s, Walk& w, bool general, Counters& cn)
+// One row: returns true when the walk has finished.  A leaf whose box is hit stops the
+// walk (w.leaf >= 0) without testing the triangle.  `general` is true when the caller
+// knows that no lane walking with it has a special ray.
+template <bool C> ZGL_D bool walkRow(const DevScene& s, Walk& w, bool general, Counters& cn)
 {
     const float4* row = rowPtr(s, w.k);
     const float4 lo = row[0];
@@ -207,18 +210,29 @@ template <bool C> ZGL_D bool walkStep(const DevScene& s, Walk& w, bool general, 
     } else {
         const int prim = __float_as_int(hi.z);
         if (prim >= 0) {
-            vec3 a, b, c;
-            loadTri(s, prim, a, b, c);
-            if (C) { cn.tests++; cn.bytes += 48; }
-            float t;
-            if (intersectTriangle(a, b, c, w.p.o, w.p.d, t) && t < w.dist) {
-                w.closest = prim;
-                if (w.any) return true;
-                w.dist = t;
-            }
+            w.leaf = prim;
+            return false;
         }
         w.k++;
     }
+    return w.k == w.end;
+}
+// The triangle test of the leaf the walk is stopped at; returns true when the walk has
+// finished.
+template <bool C> ZGL_D bool walkLeaf(const DevScene& s, Walk& w, Counters& cn)
+{
+    const int prim = w.leaf;
+    w.leaf = -1;
+    vec3 a, b, c;
+    loadTri(s, prim, a, b, c);
+    if (C) { cn.tests++; cn.bytes += 48; }
+    float t;
+    if (intersectTriangle(a, b, c, w.p.o, w.p.d, t) && t < w.dist) {
+        w.closest = prim;
+        if (w.any) return true;
+        w.dist = t;
+    }
+    w.k++;
     return w.k == w.end;
 }
 

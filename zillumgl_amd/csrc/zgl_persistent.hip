@@ -32,7 +32,8 @@ enum : int {
     kWait,      // batch: the wave's scene has no pixels left; wait for the next scene
 };
 
-// MTBVH rows each walking lane advances between two ballots of the walking loop.
+// MTBVH rows each walking lane advances per round of the walking loop (one leaf test
+// and one ballot per round).
 constexpr int kRowsPerRound = 4;
 
 // Pixel order for the dynamic fetch: 8x8 tiles in row-major tile order.
@@ -336,14 +337,22 @@ __device__ __forceinline__ void naiveBody(const DevScene& s0, const PassParams& 
         if (__ballot(phase != kDone) == 0ull) break;
 
         // ------------------------------------------------------------ walking
+        // Each round advances every walking lane by up to kRowsPerRound rows; a lane
+        // that reaches a leaf whose box is hit waits there, and the waiting lanes test
+        // their triangles together once per round, so that the triangle test (as long
+        // as the box test) runs once per round instead of on nearly every row step for
+        // a few lanes.
         {
             const bool general = __ballot(walking && wk.p.special) == 0ull;
             for (;;) {
 #pragma unroll
                 for (int u = 0; u < kRowsPerRound; ++u) {
-                    if (walking) {
-                        if (walkStep<C>(s, wk, general, cn)) walking = false;
+                    if (walking && wk.leaf < 0) {
+                        if (walkRow<C>(s, wk, general, cn)) walking = false;
                     }
+                }
+                if (walking && wk.leaf >= 0) {
+                    if (walkLeaf<C>(s, wk, cn)) walking = false;
                 }
                 if (__popcll(__ballot(walking)) <= threshold) break;
             }
