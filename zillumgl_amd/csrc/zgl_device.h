@@ -139,30 +139,41 @@ ZGL_D int traverse(const DevScene& s, const RayPre& p, float& dist, Counters& cn
     int k = p.table * s.bvhSize;
     const int end = k + s.bvhSize;
     while (k != end) {
-        const float4* row = rowPtr(s, k);
-        const float4 lo = row[0];
-        const float4 hi = row[1];
-        if (C) { cn.rows++; cn.bytes += 36; }
-        float boxDist;
-        const vec3 bMin = v3(lo.x, lo.y, lo.z), bMax = v3(lo.w, hi.x, hi.y);
-        const bool bHit = anySpecial ? boxHit(bMin, bMax, p, boxDist) : boxHitGeneral(bMin, bMax, p, boxDist);
-        if (!bHit || boxDist > dist) {
-            k = __float_as_int(hi.w);
-            continue;
+        // up to 4 rows, stopping at a leaf whose box is hit; then that leaf's triangle.
+        // Lanes of a wave reach leaves at different rows: testing once per round keeps
+        // the triangle test from running on nearly every row for a few lanes.
+        int leaf = -1;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            if (k != end && leaf < 0) {
+                const float4* row = rowPtr(s, k);
+                const float4 lo = row[0];
+                const float4 hi = row[1];
+                if (C) { cn.rows++; cn.bytes += 36; }
+                float boxDist;
+                const vec3 bMin = v3(lo.x, lo.y, lo.z), bMax = v3(lo.w, hi.x, hi.y);
+                const bool bHit = anySpecial ? boxHit(bMin, bMax, p, boxDist) : boxHitGeneral(bMin, bMax, p, boxDist);
+                if (!bHit || boxDist > dist) {
+                    k = __float_as_int(hi.w);
+                } else {
+                    const int prim = __float_as_int(hi.z);
+                    if (prim >= 0) leaf = prim;
+                    else k++;
+                }
+            }
         }
-        const int prim = __float_as_int(hi.z);
-        if (prim >= 0) {
+        if (leaf >= 0) {
             vec3 a, b, c;
-            loadTri(s, prim, a, b, c);
+            loadTri(s, leaf, a, b, c);
             if (C) { cn.tests++; cn.bytes += 48; }
             float t;
             if (intersectTriangle(a, b, c, p.o, p.d, t) && t < dist) {
-                if (ANY) return prim;
+                if (ANY) return leaf;
                 dist = t;
-                closest = prim;
+                closest = leaf;
             }
+            k++;
         }
-        k++;
     }
     return closest;
 }
