@@ -160,6 +160,14 @@ typedef struct zgl_pass_params {
 zgl_status zgl_render(zgl_context* ctx, const zgl_pass_params* params);
 /* Zero the frame (RGBA32F) and the light-splat film (planar R32F, 3W x H). */
 zgl_status zgl_clear_frame(zgl_context* ctx);
+/* Display transform of the frame (post_proc.glsl:1-59, called by Application::integrate,
+ * src/Application.cpp:645-660, without the GUI's preview mode): sum * result_scale
+ * clamped to [0, 1e30], tone mapped (0 none, 1 filmic, 2 ACES), raised to 1/2.2.
+ * rgba_out (W*H*4 floats, alpha 1) is uOut; rgb8_out (W*H*3 bytes, rows top-down) is
+ * what captureImage (Application.cpp:370-378) writes to PNG: each channel clamped to
+ * [0, 1] and rounded to round(255 x).  Either output may be NULL. */
+zgl_status zgl_post_process(zgl_context* ctx, float result_scale, int32_t tone_mapper, float* rgba_out,
+                            int64_t n_rgba, uint8_t* rgb8_out, int64_t n_rgb8);
 /* Copy the RGBA32F frame (W*H*4 floats; the running sum of the naive integrator,
  * the film copy for light/triple). Synchronous. */
 zgl_status zgl_read_frame(zgl_context* ctx, float* rgba, int64_t n_floats);

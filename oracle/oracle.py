@@ -82,6 +82,7 @@ def lib():
         L.orc_replay_loss64.argtypes = [C.POINTER(OrcScene), C.POINTER(OrcParams), _fp, _fp, _fp, C.c_int,
                                         C.c_int, C.c_int, C.c_int, C.c_int]
         L.orc_replay_loss64.restype = C.c_double
+        L.orc_post_process.argtypes = [_fp, C.c_int, C.c_int, C.c_float, C.c_int, _fp, C.POINTER(C.c_uint8)]
         L.orc_box_hit.argtypes = [_fp, _fp, C.c_int64, C.POINTER(C.c_uint8), _fp]
         L.orc_trace_closest.argtypes = [C.POINTER(OrcScene), _fp, C.c_int64, _ip, _fp, _up, _u64, C.c_int]
         L.orc_render_pixels.argtypes = [C.POINTER(OrcScene), C.POINTER(OrcParams), C.c_int, C.c_int, C.c_int,
@@ -314,6 +315,17 @@ def replay_loss64(ps, prm, eval_mat, eval_env, d_frame, x0=0, y0=0, w=None, h=No
     s = scene_struct(ps)
     return lib().orc_replay_loss64(C.byref(s), C.byref(prm), _p(em, C.c_float), _p(ee, C.c_float),
                                    _p(df, C.c_float), x0, y0, w, h, nthreads)
+
+
+def post_process(frame_rgba, scale, tone):
+    """(rgba float32 (h, w, 4), rgb8 uint8 (h, w, 3) with rows top-down)."""
+    f = np.ascontiguousarray(frame_rgba, np.float32)
+    h, w = f.shape[:2]
+    out = np.zeros((h, w, 4), np.float32)
+    rgb = np.zeros((h, w, 3), np.uint8)
+    assert lib().orc_post_process(_p(f, C.c_float), w, h, scale, tone, _p(out, C.c_float),
+                                  rgb.ctypes.data_as(C.POINTER(C.c_uint8))) == 0
+    return out, rgb
 
 
 def bsdf_sample(mat_record, wo, n, u, mode=0, seed=1):

@@ -2279,6 +2279,41 @@ int orc_render_pixels(const orc_scene* s, const orc_params* p, int x0, int y0, i
     return 0;
 }
 
+// post_proc.glsl:1-59 and the RGB8 read-back of captureImage (Application.cpp:370-378):
+// sum * scale, clamp(0, 1e30), tone map (0 none, 1 filmic, 2 ACES), pow(1/2.2); bytes
+// are round(255 * clamp(x, 0, 1)) with rows written top-down.
+static float filmicCalc(float x)
+{
+    const float A = 0.22f, B = 0.3f, C = 0.1f, D = 0.2f, E = 0.01f, F = 0.3f;
+    const float bc = B * C, de = D * E, df = D * F, ef = E / F;
+    return (x * (x * A + bc) + de) / (x * (x * A + B) + df) - ef;
+}
+int orc_post_process(const float* frame, int w, int h, float scale, int tone, float* out, uint8_t* rgb8)
+{
+    if (!frame || w < 1 || h < 1) return 1;
+    const float g = 1.0f / 2.2f;
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++) {
+            const float* f = frame + 4 * ((size_t)y * w + x);
+            float m[3];
+            for (int c = 0; c < 3; c++) {
+                float v = gmin(gmax(f[c] * scale, 0.0f), 1e30f);
+                if (tone == 1) v = filmicCalc(v * 1.6f) / filmicCalc(11.2f);
+                else if (tone == 2) v = (v * (v * 2.51f + 0.03f)) / (v * (v * 2.43f + 0.59f) + 0.14f);
+                m[c] = powF(v, g);
+            }
+            if (out) {
+                float* o = out + 4 * ((size_t)y * w + x);
+                o[0] = m[0]; o[1] = m[1]; o[2] = m[2]; o[3] = 1.0f;
+            }
+            if (rgb8)
+                for (int c = 0; c < 3; c++)
+                    rgb8[3 * ((size_t)(h - 1 - y) * w + x) + c] =
+                        std::isnan(m[c]) ? 0 : (uint8_t)(int)(gmin(gmax(m[c], 0.0f), 1.0f) * 255.0f + 0.5f);
+        }
+    return 0;
+}
+
 // Differentiable naive estimator (no counterpart in the reference): the replay
 // estimator's frame sums over a region (FP32), and the FP64 gradient of
 // sum(d_frame . frame) w.r.t. the material parameters (n x 5: baseColor rgb, raw

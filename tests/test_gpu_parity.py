@@ -273,3 +273,23 @@ def test_render_rejects_bad_threshold(ctx, c1):
         ctx.render(_lib.INTEGRATOR_NAIVE, 0, 1, 1, 4, regen_threshold=64)
     with pytest.raises(_lib.ZglError):
         ctx.render(_lib.INTEGRATOR_NAIVE, 0, 1, 1, 4, regen_threshold=-1)
+
+
+@pytest.mark.parametrize("tone", [0, 1, 2])
+def test_post_process_matches_oracle(ctx, c1, tone):
+    """post_proc.glsl on the GPU == the oracle bit for bit (floats and RGB8 bytes), on a
+    rendered frame with extreme values written into a few pixels."""
+    print(f"[gpu] post-process tone {tone}", flush=True)
+    ctx.upload(c1)
+    ctx.clear()
+    ctx.render(_lib.INTEGRATOR_NAIVE, 0, 4, 1, 5)
+    f = ctx.read_frame().copy()
+    f[0, 0, :3] = [-1.0, 0.0, 1e30]
+    f[0, 1, :3] = [3e38, 1e-30, 0.5]
+    f[1, 0, :3] = [np.inf, 2.0, 1e-7]
+    ctx.write_frame(f)
+    rgba, rgb = ctx.post_process(np.float32(1.0 / 5.0), tone)
+    ref_rgba, ref_rgb = orc.post_process(f, np.float32(1.0 / 5.0), tone)
+    assert np.array_equal(rgba.view(np.uint32), ref_rgba.view(np.uint32))
+    assert np.array_equal(rgb, ref_rgb)
+    assert 0 < rgb.mean() < 255

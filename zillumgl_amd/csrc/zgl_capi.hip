@@ -673,6 +673,29 @@ zgl_status zgl_adjoint(zgl_context* ctx, const zgl_pass_params* p, const float* 
     });
 }
 
+zgl_status zgl_post_process(zgl_context* ctx, float result_scale, int32_t tone_mapper, float* rgba_out,
+                            int64_t n_rgba, uint8_t* rgb8_out, int64_t n_rgb8)
+{
+    return guarded(ctx, [&] {
+        if (!ctx->hasCamera) throw ZglError{ZGL_E_STATE, "no camera"};
+        if (tone_mapper < 0 || tone_mapper > 2) invalid("tone_mapper must be 0 (none), 1 (filmic) or 2 (ACES)");
+        const int64_t px = (int64_t)ctx->filmW * ctx->filmH;
+        if (rgba_out && n_rgba < px * 4) invalid("rgba buffer too small");
+        if (rgb8_out && n_rgb8 < px * 3) invalid("rgb8 buffer too small");
+        ctx->ensure(ctx->scratchOut, (size_t)px * 19, "post-process output");
+        float4* dOut = (float4*)ctx->scratchOut.p;
+        unsigned char* dRgb = (unsigned char*)ctx->scratchOut.p + (size_t)px * 16;
+        hipCheck(launchPostProcess((const float4*)ctx->frame.p, rgba_out ? dOut : nullptr, rgb8_out ? dRgb : nullptr,
+                                   ctx->filmW, ctx->filmH, result_scale, tone_mapper, ctx->stream),
+                 "post-process");
+        if (rgba_out)
+            hipCheck(hipMemcpyAsync(rgba_out, dOut, (size_t)px * 16, hipMemcpyDeviceToHost, ctx->stream), "copy");
+        if (rgb8_out)
+            hipCheck(hipMemcpyAsync(rgb8_out, dRgb, (size_t)px * 3, hipMemcpyDeviceToHost, ctx->stream), "copy");
+        hipCheck(hipStreamSynchronize(ctx->stream), "post-process");
+    });
+}
+
 zgl_status zgl_clear_frame(zgl_context* ctx)
 {
     return guarded(ctx, [&] {

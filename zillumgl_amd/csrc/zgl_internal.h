@@ -38,6 +38,8 @@ struct BatchScene {
 
 hipError_t launchSobolTable(const uint32_t* sobolM, uint32_t* tab, int first, int count, hipStream_t st);
 hipError_t launchFill(float* p, int64_t n, float v, hipStream_t st);
+hipError_t launchPostProcess(const float4* frame, float4* out, unsigned char* rgb8, int w, int h, float scale, int tone,
+                             hipStream_t st);
 hipError_t launchFilmToFrame(const float* film, float4* frame, int w, int h, hipStream_t st);
 hipError_t launchTraceClosest(const DevScene& s, const float* rays, int64_t n, int32_t* prim, float* t,
                               uint32_t* bin, unsigned long long* counters, hipStream_t st);

@@ -7,10 +7,12 @@
 //   k_triple_pt       triple_path_pass_pt.glsl:44-224
 //   k_triple_lpt      triple_path_pass_lpt.glsl:36-193
 //   k_film_to_frame   util/img_copy_1x32f_4x32f.glsl
+//   k_post_process    post_proc.glsl + the RGB8 read-back of captureImage
 // A launch may render several consecutive passes; per-pixel sums are then carried in
 // registers in pass order, which is the same sequence of float additions as one
 // launch per pass.
 #include "zgl_internal.h"
+#include "zgl_post.h"
 
 namespace zgl {
 
@@ -64,6 +66,25 @@ __global__ void k_fill(float* p, int64_t n, float v)
 {
     int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     for (; g < n; g += (int64_t)gridDim.x * blockDim.x) p[g] = v;
+}
+
+// post_proc.glsl:43-58: frame sum -> display colour (RGBA32F, alpha 1) and/or RGB8
+// rows flipped top-down (captureImage: stbi_flip_vertically_on_write).
+__global__ void k_post_process(const float4* __restrict__ frame, float4* __restrict__ out,
+                               unsigned char* __restrict__ rgb8, int w, int h, float scale, int tone)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= w * h) return;
+    const int x = g % w, y = g / w;
+    const float4 f = frame[g];
+    const vec3 m = displayColor(v3(f.x, f.y, f.z), scale, tone);
+    if (out) out[g] = make_float4(m.x, m.y, m.z, 1.0f);
+    if (rgb8) {
+        unsigned char* o = rgb8 + ((size_t)(h - 1 - y) * w + x) * 3;
+        o[0] = toUnorm8(m.x);
+        o[1] = toUnorm8(m.y);
+        o[2] = toUnorm8(m.z);
+    }
 }
 
 __global__ void k_film_to_frame(const float* __restrict__ film, float4* __restrict__ frame, int w, int h)
@@ -499,6 +520,15 @@ hipError_t launchFill(float* p, int64_t n, float v, hipStream_t st)
     hipLaunchKernelGGL(k_fill, dim3(g), dim3(256), 0, st, p, n, v);
     return hipGetLastError();
 }
+hipError_t launchPostProcess(const float4* frame, float4* out, unsigned char* rgb8, int w, int h, float scale, int tone,
+                             hipStream_t st)
+{
+    const int n = w * h;
+    hipLaunchKernelGGL(k_post_process, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, frame, out, rgb8, w, h,
+                       scale, tone);
+    return hipGetLastError();
+}
+
 hipError_t launchFilmToFrame(const float* film, float4* frame, int w, int h, hipStream_t st)
 {
     hipLaunchKernelGGL(k_film_to_frame, dim3(gridFor((int64_t)w * h, 256)), dim3(256), 0, st, film, frame, w, h);

@@ -139,6 +139,16 @@ class Context:
         self._check(_lib.lib().zgl_read_frame(self.h, ptr(out, C.c_float), out.size))
         return out
 
+    def post_process(self, result_scale, tone_mapper=1):
+        """The display transform of post_proc.glsl: (rgba float32 (H, W, 4), rgb8 (H, W, 3)
+        rows top-down, as captureImage writes them).  tone_mapper 0 none, 1 filmic
+        (the reference's default), 2 ACES."""
+        rgba = np.empty((self.height, self.width, 4), np.float32)
+        rgb = np.empty((self.height, self.width, 3), np.uint8)
+        self._check(_lib.lib().zgl_post_process(self.h, result_scale, tone_mapper, ptr(rgba, C.c_float), rgba.size,
+                                                ptr(rgb, C.c_uint8), rgb.size))
+        return rgba, rgb
+
     def read_film(self):
         out = np.empty((self.height, self.width * 3), np.float32)
         self._check(_lib.lib().zgl_read_film(self.h, ptr(out, C.c_float), out.size))
