@@ -213,20 +213,19 @@ template <bool C> ZGL_D bool walkRow(const DevScene& s, Walk& w, bool general, C
     const float4 lo = row[0];
     const float4 hi = row[1];
     if (C) { cn.rows++; cn.bytes += 36; }
-    float boxDist;
     const vec3 bMin = v3(lo.x, lo.y, lo.z), bMax = v3(lo.w, hi.x, hi.y);
-    const bool bHit = general ? boxHitGeneral(bMin, bMax, w.p, boxDist) : boxHit(bMin, bMax, w.p, boxDist);
-    if (!bHit || boxDist > w.dist) {
-        w.k = __float_as_int(hi.w);
-    } else {
-        const int prim = __float_as_int(hi.z);
-        if (prim >= 0) {
-            w.leaf = prim;
-            return false;
-        }
-        w.k++;
-    }
-    return w.k == w.end;
+    // the three-slab test straight after its loads; the other cases only when a lane of
+    // the wave needs them
+    float boxDist;
+    bool bHit = boxHitGeneral(bMin, bMax, w.p, boxDist);
+    if (!general) bHit = boxHit(bMin, bMax, w.p, boxDist);
+    // next row: the miss link, the row after, or stay at a hit leaf (selects, no branch)
+    const bool take = bHit && !(boxDist > w.dist);
+    const int prim = __float_as_int(hi.z);
+    const bool leaf = take && prim >= 0;
+    w.leaf = leaf ? prim : w.leaf;
+    w.k = take ? (leaf ? w.k : w.k + 1) : __float_as_int(hi.w);
+    return !leaf && w.k == w.end;
 }
 // The triangle test of the leaf the walk is stopped at; returns true when the walk has
 // finished.
