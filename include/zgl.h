@@ -84,7 +84,9 @@ typedef struct zgl_scene_buffers {
     const int32_t* indices;       /* 3 per triangle: objects first, then lights */
     int64_t n_triangles;
     const float* bounds;          /* bvh_size nodes x {pMin.xyz, pMax.xyz} */
-    const int32_t* hit_table;     /* 6 x bvh_size x {node, prim, miss} */
+    const int32_t* hit_table;     /* 6 x bvh_size x {node, prim, miss}; bounds and
+                                     hit_table both NULL: the MTBVH is built on the
+                                     device (as zgl_build_mtbvh_device) */
     int32_t bvh_size;             /* boxCount = 2n - 1 */
     const int32_t* mat_tex_indices; /* one per object triangle */
     int32_t obj_prim_count;       /* objPrimCount */
@@ -233,6 +235,39 @@ zgl_status zgl_batch_synchronize(zgl_batch* batch);
 zgl_status zgl_batch_tune(zgl_batch* batch, int32_t reps, double* best_ms);
 zgl_status zgl_batch_get_config(const zgl_batch* batch, int32_t* regen_threshold, int32_t* blocks_per_cu,
                                 int32_t* order, int32_t max_blocks_per_cu[1]);
+
+/* ------------------------------------------------------------------ */
+/* MTBVH construction on the device                                     */
+/* ------------------------------------------------------------------ */
+/* The same output as zgl_build_mtbvh, bit for bit -- BVH::build + buildHitTable
+ * (src/accelerator/BVH.cpp:116-345), the reference's CPU build, whose GPU version its
+ * README lists as future work (README.md:24) -- built by HIP kernels: node bounds
+ * ((2n-1)*6 floats) and six hit tables (6*(2n-1)*3 ints).  Vertices must be finite.
+ * Scenes whose triangle centroids reach 1e8 in magnitude (the reference's empty-box
+ * value) can make a split leave a child empty; both builds reject them with
+ * ZGL_E_INVALID.  A builder keeps its device buffers between builds and is not
+ * thread-safe. */
+typedef struct zgl_bvh_builder zgl_bvh_builder;
+zgl_status zgl_bvh_builder_create(int32_t device, zgl_bvh_builder** out);
+void zgl_bvh_builder_destroy(zgl_bvh_builder* builder);
+const char* zgl_bvh_builder_last_error(const zgl_bvh_builder* builder);
+/* Copy a mesh (n_vertices x 3 floats, n_triangles x 3 indices, 1 <= n_triangles <= 2^27)
+ * to the device. */
+zgl_status zgl_bvh_builder_upload(zgl_bvh_builder* builder, const float* vertices, int64_t n_vertices,
+                                  const uint32_t* indices, int64_t n_triangles);
+/* Build from the uploaded mesh; returns when the build is complete.  subtree_size: nodes
+ * with fewer triangles are finished by one wave each (0 = default 256, minimum 3); the
+ * output does not depend on it. */
+zgl_status zgl_bvh_builder_build(zgl_bvh_builder* builder, int32_t subtree_size);
+/* Copy the last build's bounds and hit tables (either may be NULL). */
+zgl_status zgl_bvh_builder_download(zgl_bvh_builder* builder, float* bounds_out, int32_t* hit_table_out);
+/* Statistics of the last build: [0] levels of the multi-workgroup phase, [1] nodes split
+ * there, [2] subtrees finished by one wave, [3] tree depth, [4] pointer-doubling rounds. */
+zgl_status zgl_bvh_builder_stats(const zgl_bvh_builder* builder, int64_t stats_out[5]);
+/* One call: upload, build and download on `device`. */
+zgl_status zgl_build_mtbvh_device(int32_t device, const float* vertices, int64_t n_vertices,
+                                  const uint32_t* indices, int64_t n_triangles, float* bounds_out,
+                                  int32_t* hit_table_out);
 
 #ifdef __cplusplus
 }

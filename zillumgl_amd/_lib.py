@@ -102,6 +102,14 @@ SIGNATURES = [
     ("zgl_batch_synchronize", C.c_int32, [C.c_void_p]),
     ("zgl_batch_tune", C.c_int32, [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
     ("zgl_batch_get_config", C.c_int32, [C.c_void_p, _ip, _ip, _ip, _ip]),
+    ("zgl_bvh_builder_create", C.c_int32, [C.c_int32, C.POINTER(C.c_void_p)]),
+    ("zgl_bvh_builder_destroy", None, [C.c_void_p]),
+    ("zgl_bvh_builder_last_error", C.c_char_p, [C.c_void_p]),
+    ("zgl_bvh_builder_upload", C.c_int32, [C.c_void_p, _fp, C.c_int64, _up, C.c_int64]),
+    ("zgl_bvh_builder_build", C.c_int32, [C.c_void_p, C.c_int32]),
+    ("zgl_bvh_builder_download", C.c_int32, [C.c_void_p, _fp, _ip]),
+    ("zgl_bvh_builder_stats", C.c_int32, [C.c_void_p, C.POINTER(C.c_int64)]),
+    ("zgl_build_mtbvh_device", C.c_int32, [C.c_int32, _fp, C.c_int64, _up, C.c_int64, _fp, _ip]),
 ]
 
 _lib = None
@@ -151,6 +159,19 @@ def build_mtbvh(vertices, indices):
     table = np.empty((6, 2 * n - 1, 3), np.int32)
     check(lib().zgl_build_mtbvh(ptr(v, C.c_float), v.shape[0], ptr(i, C.c_uint32), n,
                                 ptr(bounds, C.c_float), ptr(table, C.c_int32)), what="zgl_build_mtbvh")
+    return bounds, table
+
+
+def build_mtbvh_device(vertices, indices, device=0):
+    """The same MTBVH as build_mtbvh, built by the GPU kernels (zgl_build_mtbvh_device)."""
+    v = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 3)
+    i = np.ascontiguousarray(indices, dtype=np.uint32).reshape(-1, 3)
+    n = i.shape[0]
+    bounds = np.empty(((2 * n - 1), 6), np.float32)
+    table = np.empty((6, 2 * n - 1, 3), np.int32)
+    check(lib().zgl_build_mtbvh_device(device, ptr(v, C.c_float), v.shape[0], ptr(i, C.c_uint32), n,
+                                       ptr(bounds, C.c_float), ptr(table, C.c_int32)),
+          what="zgl_build_mtbvh_device")
     return bounds, table
 
 

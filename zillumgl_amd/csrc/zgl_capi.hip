@@ -9,6 +9,7 @@
 
 #include "../../include/zgl.h"
 #include "zgl_build.h"
+#include "zgl_bvh.h"
 #include "zgl_internal.h"
 
 using namespace zgl;
@@ -73,6 +74,8 @@ struct zgl_context {
     DevBuf frame, film, counters, scratchRays, scratchOut, workCounter;
     // differentiable mode: evaluation point, adjoint input and outputs
     DevBuf evalMat, evalEnv, dFrame, dMat, dEnv;
+    DevBuf bvhBounds;  // node bounds of a device-built MTBVH
+    DeviceBvh bvh;     // work memory of the device MTBVH build
     std::vector<float> hostMaterials;  // uploaded material records (16 floats each)
     int numCUs = 0;
     int naiveBlocksPerCU = 0;
@@ -87,8 +90,9 @@ struct zgl_context {
         DevBuf* all[] = {&rows, &tris, &indices, &vertices, &normals, &texcoords, &matTex, &materials,
                          &lightPower, &lightAlias, &lightProb, &env, &envAlias, &envProb, &noise, &sobolM,
                          &sobolTab, &frame, &film, &counters, &scratchRays, &scratchOut, &workCounter,
-                         &evalMat, &evalEnv, &dFrame, &dMat, &dEnv};
+                         &evalMat, &evalEnv, &dFrame, &dMat, &dEnv, &bvhBounds};
         for (DevBuf* b : all) b->release();
+        bvh.release();
     }
 };
 
@@ -283,7 +287,9 @@ zgl_status zgl_upload_scene(zgl_context* ctx, const zgl_scene_buffers* sb)
         if (nV < 3 || nT < 1 || !sb->vertices || !sb->normals || !sb->indices) invalid("empty geometry");
         // absolute MTBVH row indices (6 tables x (2 nT - 1) rows) must fit in int32
         if (nT > (1LL << 27) || nV > (1LL << 30)) invalid("scene too large");
-        if (sb->bvh_size != 2 * nT - 1 || !sb->bounds || !sb->hit_table) invalid("bvh_size must be 2*n_triangles-1");
+        const bool deviceBvh = !sb->bounds && !sb->hit_table;
+        if (sb->bvh_size != 2 * nT - 1) invalid("bvh_size must be 2*n_triangles-1");
+        if (!deviceBvh && (!sb->bounds || !sb->hit_table)) invalid("bounds and hit_table: both or neither");
         if (sb->obj_prim_count < 0 || sb->obj_prim_count > nT) invalid("obj_prim_count out of range");
         if (sb->n_light_triangles != nT - sb->obj_prim_count) invalid("n_light_triangles != n_triangles - obj_prim_count");
         if (sb->n_texcoords < 0 || sb->n_texcoords > nV || (sb->n_texcoords > 0 && !sb->texcoords))
@@ -297,11 +303,12 @@ zgl_status zgl_upload_scene(zgl_context* ctx, const zgl_scene_buffers* sb)
         for (int64_t i = 0; i < sb->obj_prim_count; i++)
             if ((sb->mat_tex_indices[i] & 0xffff) >= sb->n_materials) invalid("material index out of range");
         const int64_t B = sb->bvh_size;
-        // Finite bounds: the box test of general rays relies on NaN-free slab distances.
-        for (int64_t i = 0; i < B * 6; i++)
+        // Finite bounds: the box test of general rays relies on NaN-free slab distances
+        // (the device build checks the vertices instead).
+        for (int64_t i = 0; !deviceBvh && i < B * 6; i++)
             if (!(std::fabs(sb->bounds[i]) <= 3.4028235e38f)) invalid("non-finite MTBVH bounds");
         // Every row must point forward so that every traversal terminates.
-        for (int t = 0; t < 6; t++)
+        for (int t = 0; t < 6 && !deviceBvh; t++)
             for (int64_t k = 0; k < B; k++) {
                 const int32_t* r = sb->hit_table + (t * B + k) * 3;
                 if (r[0] < 0 || r[0] >= B || r[1] < -1 || r[1] >= nT || r[2] <= k || r[2] > B)
@@ -325,8 +332,8 @@ zgl_status zgl_upload_scene(zgl_context* ctx, const zgl_scene_buffers* sb)
 
         // MTBVH row records: {pMin.xyz, pMax.x} {pMax.yz, prim, miss}, the miss link
         // made absolute (table * B + miss) so that a walk needs no per-table base
-        std::vector<float> rows((size_t)6 * B * 8);
-        for (int t = 0; t < 6; t++)
+        std::vector<float> rows(deviceBvh ? 0 : (size_t)6 * B * 8);
+        for (int t = 0; t < 6 && !deviceBvh; t++)
             for (int64_t k = 0; k < B; k++) {
                 const int32_t* r = sb->hit_table + (t * B + k) * 3;
                 const float* bb = sb->bounds + (size_t)r[0] * 6;
@@ -347,10 +354,20 @@ zgl_status zgl_upload_scene(zgl_context* ctx, const zgl_scene_buffers* sb)
         for (size_t i = 0; i < (size_t)w * h; i++)
             for (int c = 0; c < 3; c++) env[i * 4 + c] = sb->env_rgb16f[i * 3 + c];
 
-        ctx->upload(ctx->rows, rows.data(), rows.size() * 4, "rows");
+        if (!deviceBvh) ctx->upload(ctx->rows, rows.data(), rows.size() * 4, "rows");
         ctx->upload(ctx->tris, tris.data(), tris.size() * 4, "triangles");
         ctx->upload(ctx->indices, sb->indices, (size_t)nT * 12, "indices");
         ctx->upload(ctx->vertices, sb->vertices, (size_t)nV * 12, "vertices");
+        if (deviceBvh) {
+            // indices were checked to lie in [0, n_vertices): the int32 buffer reads as uint32
+            ctx->ensure(ctx->rows, (size_t)6 * B * 32, "rows");
+            ctx->ensure(ctx->bvhBounds, (size_t)B * 24, "MTBVH bounds");
+            std::string msg;
+            const int rc = ctx->bvh.build((const float*)ctx->vertices.p, nV, (const uint32_t*)ctx->indices.p, nT,
+                                          kDefaultBvhSubtree, (float*)ctx->bvhBounds.p, nullptr, ctx->rows.p,
+                                          ctx->stream, msg);
+            if (rc) throw ZglError{rc == 1 ? ZGL_E_INVALID : (rc == 3 ? ZGL_E_OOM : ZGL_E_HIP), msg};
+        }
         ctx->upload(ctx->normals, sb->normals, (size_t)nV * 12, "normals");
         ctx->upload(ctx->texcoords, sb->texcoords, (size_t)sb->n_texcoords * 8, "texcoords");
         ctx->upload(ctx->matTex, sb->mat_tex_indices, (size_t)sb->obj_prim_count * 4, "matTexIndices");
@@ -988,6 +1005,130 @@ zgl_status zgl_batch_get_config(const zgl_batch* b, int32_t* regen_threshold, in
     if (order) std::memcpy(order, b->order.data(), b->order.size() * 4);
     if (max_blocks_per_cu) max_blocks_per_cu[0] = b->maxBlocksPerCU;
     return ZGL_OK;
+}
+
+// ---------------------------------------------------------------- device MTBVH build
+}  // extern "C"
+
+struct zgl_bvh_builder {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    DevBuf vertices, indices, bounds, hit;
+    int64_t nV = 0, nT = 0;
+    bool built = false;
+    DeviceBvh bvh;
+    ~zgl_bvh_builder()
+    {
+        DevBuf* all[] = {&vertices, &indices, &bounds, &hit};
+        for (DevBuf* b : all) b->release();
+        bvh.release();
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+extern "C" {
+
+zgl_status zgl_bvh_builder_create(int32_t device, zgl_bvh_builder** out)
+{
+    if (!out) return ZGL_E_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1) return ZGL_E_NODEVICE;
+    if (device < 0 || device >= n) return ZGL_E_INVALID;
+    zgl_bvh_builder* b = new (std::nothrow) zgl_bvh_builder;
+    if (!b) return ZGL_E_OOM;
+    b->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete b;
+        return ZGL_E_HIP;
+    }
+    *out = b;
+    return ZGL_OK;
+}
+
+void zgl_bvh_builder_destroy(zgl_bvh_builder* b)
+{
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    delete b;
+}
+
+const char* zgl_bvh_builder_last_error(const zgl_bvh_builder* b) { return b ? b->err.c_str() : "null builder"; }
+
+zgl_status zgl_bvh_builder_upload(zgl_bvh_builder* b, const float* vertices, int64_t n_vertices,
+                                  const uint32_t* indices, int64_t n_triangles)
+{
+    return guarded(b, [&] {
+        if (!vertices || !indices || n_vertices < 1 || n_triangles < 1) invalid("empty mesh");
+        if (n_triangles > (1LL << 27) || n_vertices > (1LL << 30)) invalid("mesh too large");
+        bufUpload(b->vertices, vertices, (size_t)n_vertices * 12, b->stream, "vertices");
+        bufUpload(b->indices, indices, (size_t)n_triangles * 12, b->stream, "indices");
+        hipCheck(hipStreamSynchronize(b->stream), "mesh upload");
+        b->nV = n_vertices;
+        b->nT = n_triangles;
+        b->built = false;
+    });
+}
+
+zgl_status zgl_bvh_builder_build(zgl_bvh_builder* b, int32_t subtree_size)
+{
+    return guarded(b, [&] {
+        if (b->nT < 1) throw ZglError{ZGL_E_STATE, "build before upload"};
+        if (subtree_size < 0) invalid("subtree_size");
+        const int64_t B = 2 * b->nT - 1;
+        bufEnsure(b->bounds, (size_t)B * 24, "MTBVH bounds");
+        bufEnsure(b->hit, (size_t)B * 72, "hit tables");
+        b->built = false;
+        std::string msg;
+        const int rc = b->bvh.build((const float*)b->vertices.p, b->nV, (const uint32_t*)b->indices.p, b->nT,
+                                    subtree_size ? subtree_size : kDefaultBvhSubtree, (float*)b->bounds.p,
+                                    (int32_t*)b->hit.p, nullptr, b->stream, msg);
+        if (rc) throw ZglError{rc == 1 ? ZGL_E_INVALID : (rc == 3 ? ZGL_E_OOM : ZGL_E_HIP), msg};
+        hipCheck(hipStreamSynchronize(b->stream), "MTBVH build");
+        b->built = true;
+    });
+}
+
+zgl_status zgl_bvh_builder_download(zgl_bvh_builder* b, float* bounds_out, int32_t* hit_table_out)
+{
+    return guarded(b, [&] {
+        if (!b->built) throw ZglError{ZGL_E_STATE, "download before a successful build"};
+        const int64_t B = 2 * b->nT - 1;
+        if (bounds_out)
+            hipCheck(hipMemcpyAsync(bounds_out, b->bounds.p, (size_t)B * 24, hipMemcpyDeviceToHost, b->stream),
+                     "download bounds");
+        if (hit_table_out)
+            hipCheck(hipMemcpyAsync(hit_table_out, b->hit.p, (size_t)B * 72, hipMemcpyDeviceToHost, b->stream),
+                     "download hit tables");
+        hipCheck(hipStreamSynchronize(b->stream), "download");
+    });
+}
+
+zgl_status zgl_bvh_builder_stats(const zgl_bvh_builder* b, int64_t stats_out[5])
+{
+    if (!b || !stats_out) return ZGL_E_INVALID;
+    const DeviceBvhStats& s = b->bvh.stats;
+    stats_out[0] = s.levels;
+    stats_out[1] = s.bigNodes;
+    stats_out[2] = s.smallTasks;
+    stats_out[3] = s.maxDepth;
+    stats_out[4] = s.jumps;
+    return ZGL_OK;
+}
+
+zgl_status zgl_build_mtbvh_device(int32_t device, const float* vertices, int64_t n_vertices,
+                                  const uint32_t* indices, int64_t n_triangles, float* bounds_out,
+                                  int32_t* hit_table_out)
+{
+    zgl_bvh_builder* b = nullptr;
+    zgl_status st = zgl_bvh_builder_create(device, &b);
+    if (st != ZGL_OK) return st;
+    st = zgl_bvh_builder_upload(b, vertices, n_vertices, indices, n_triangles);
+    if (st == ZGL_OK) st = zgl_bvh_builder_build(b, 0);
+    if (st == ZGL_OK) st = zgl_bvh_builder_download(b, bounds_out, hit_table_out);
+    zgl_bvh_builder_destroy(b);
+    return st;
 }
 
 }  // extern "C"

@@ -349,3 +349,56 @@ class TriplePathIntegrator:
 
     def result_scale(self):
         return 1.0 / self.sample_per_pixel
+
+
+class BvhBuilder:
+    """MTBVH construction on the GPU (zgl_bvh_builder_*): the same bounds and hit tables
+    as the host build (_lib.build_mtbvh), bit for bit."""
+
+    def __init__(self, device=0):
+        L = _lib.lib()
+        h = C.c_void_p()
+        st = L.zgl_bvh_builder_create(device, C.byref(h))
+        if st != _lib.ZGL_OK:
+            raise _lib.ZglError(st, f"zgl_bvh_builder_create(device={device}) failed")
+        self.h = h
+        self.n = 0
+
+    def close(self):
+        if self.h:
+            _lib.lib().zgl_bvh_builder_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st):
+        if st != _lib.ZGL_OK:
+            raise _lib.ZglError(st, _lib.lib().zgl_bvh_builder_last_error(self.h).decode())
+
+    def upload(self, vertices, indices):
+        v = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 3)
+        i = np.ascontiguousarray(indices, dtype=np.uint32).reshape(-1, 3)
+        self._check(_lib.lib().zgl_bvh_builder_upload(self.h, _lib.ptr(v, C.c_float), v.shape[0],
+                                                      _lib.ptr(i, C.c_uint32), i.shape[0]))
+        self.n = i.shape[0]
+
+    def build(self, subtree_size=0):
+        """Builds on the device; returns when done."""
+        self._check(_lib.lib().zgl_bvh_builder_build(self.h, subtree_size))
+
+    def download(self):
+        n = self.n
+        bounds = np.empty(((2 * n - 1), 6), np.float32)
+        table = np.empty((6, 2 * n - 1, 3), np.int32)
+        self._check(_lib.lib().zgl_bvh_builder_download(self.h, _lib.ptr(bounds, C.c_float),
+                                                        _lib.ptr(table, C.c_int32)))
+        return bounds, table
+
+    def stats(self):
+        out = (C.c_int64 * 5)()
+        self._check(_lib.lib().zgl_bvh_builder_stats(self.h, out))
+        return dict(zip(("levels", "big_nodes", "small_subtrees", "depth", "jumps"), list(out)))

@@ -194,7 +194,7 @@ class PackedScene:
         sb.n_triangles = self.indices.shape[0]
         sb.bounds = P(self.bounds, C.c_float)
         sb.hit_table = P(self.hit_table, C.c_int32)
-        sb.bvh_size = self.bounds.shape[0]
+        sb.bvh_size = 2 * self.indices.shape[0] - 1
         sb.mat_tex_indices = P(self.mat_tex_indices, C.c_int32)
         sb.obj_prim_count = self.obj_prim_count
         sb.materials = P(self.materials, C.c_float)
@@ -229,7 +229,12 @@ def _lum601(c):
     return (_f(c[0]) * _f(0.299) + _f(c[1]) * _f(0.587)) + _f(c[2]) * _f(0.114)
 
 
-def pack(desc: SceneDesc) -> PackedScene:
+def pack(desc: SceneDesc, bvh: str = "host") -> PackedScene:
+    """Scene buffers.  bvh="host" builds the MTBVH with the host build; bvh="device"
+    leaves bounds and hit tables out (None), and zgl_upload_scene then builds the MTBVH
+    on the GPU."""
+    if bvh not in ("host", "device"):
+        raise ValueError("bvh must be 'host' or 'device'")
     verts, norms, texs, inds, mti, mats = [], [], [], [], [], []
     off_v, off_m, obj_prims = 0, 0, 0
     for mesh, mat in desc.objects:
@@ -279,7 +284,10 @@ def pack(desc: SceneDesc) -> PackedScene:
     ps.materials = np.ascontiguousarray(np.stack(mats), np.float32)
     ps.obj_prim_count = obj_prims
     ps.n_light_triangles = ps.indices.shape[0] - obj_prims
-    ps.bounds, ps.hit_table = _lib.build_mtbvh(ps.vertices, ps.indices.astype(np.uint32))
+    if bvh == "host":
+        ps.bounds, ps.hit_table = _lib.build_mtbvh(ps.vertices, ps.indices.astype(np.uint32))
+    else:
+        ps.bounds, ps.hit_table = None, None
     if light_power:
         ps.light_power = np.ascontiguousarray(np.stack(light_power), np.float32)
         ps.light_alias, ps.light_prob = _lib.alias_build(np.asarray(pdf, np.float32))
