@@ -197,6 +197,42 @@ def batch_bench(reps):
                         "frames bit-identical between the two forms"}
 
 
+def bvh_bench(reps):
+    """GPU MTBVH build (zgl_bvh_builder_build) of the largest scene of the batched workload
+    (Cornell box tessellated to 1,319,508 triangles), wall clock per build including its
+    per-level synchronisations, against the library's single-threaded host build of the
+    same mesh; the two outputs are compared bit for bit."""
+    import numpy as np
+    from zillumgl_amd import scene, render, _lib
+    descs = scene.config4()
+    sizes = [sum(m.indices.shape[0] for m, _ in d.objects) + sum(m.indices.shape[0] for m, _ in d.lights)
+             for d, _, _ in descs]
+    ps = scene.pack(descs[int(np.argmax(sizes))][0], bvh="device")
+    v, idx = ps.vertices, ps.indices.astype(np.uint32)
+    n = idx.shape[0]
+    b = render.BvhBuilder(0)
+    b.upload(v, idx)
+    b.build(0)
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        b.build(0)
+        ts.append(time.perf_counter() - t0)
+    db, dt = b.download()
+    st = b.stats()
+    b.close()
+    t0 = time.perf_counter()
+    hb, ht = _lib.build_mtbvh(v, idx)
+    host_s = time.perf_counter() - t0
+    ms = float(np.median(ts)) * 1e3
+    return {"metric": "MTBVH build throughput", "value": round(n / ms / 1e3, 1), "unit": "Mtriangles/s",
+            "workload": f"largest batched-workload scene, {n} triangles", "triangles": n,
+            "device_ms": round(ms, 3), "host_ms": round(host_s * 1e3, 1), "host_threads": 1,
+            "speedup_vs_host": round(host_s * 1e3 / ms, 1),
+            "identical": bool(np.array_equal(hb.view(np.uint32), db.view(np.uint32)) and np.array_equal(ht, dt)),
+            "levels": st["levels"], "one_wave_subtrees": st["small_subtrees"], "depth": st["depth"]}
+
+
 def rmse_vs_converged(ctx, W, H, spp, depth, ref_spp):
     """RMSE of the benchmark image (mean of passes 0..spp-1) against a converged image of
     ref_spp further passes (spp..spp+ref_spp-1, independent of the image's samples) of
@@ -234,12 +270,17 @@ def main():
     ap.add_argument("--no-count", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-bvh", action="store_true")
+    ap.add_argument("--bvh-only", action="store_true", help="only the MTBVH build line (profiling)")
     ap.add_argument("--reference-spp", type=int, default=0, metavar="N",
                     help="also report the RMSE against an N-spp converged image (N a multiple of 64)")
     ap.add_argument("--threshold", type=int, default=0, help="persistent schedule regeneration threshold (0 = default)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES (PMC calibration) and exit")
     args = ap.parse_args()
+    if args.bvh_only:
+        print(json.dumps(bvh_bench(max(args.steps, 5))))
+        return
     if args.copy_probe:
         import torch
         src = torch.ones(args.copy_probe // 4, dtype=torch.float32, device="cuda")
@@ -332,6 +373,12 @@ def main():
         print(f"[bench] batch {batch['value']} Msamples/s, {batch['speedup_vs_sequential']}x sequential",
               file=sys.stderr, flush=True)
 
+    bvh = None
+    if not args.no_bvh and args.workload == "c2":
+        bvh = bvh_bench(max(args.steps, 5))
+        print(f"[bench] MTBVH build {bvh['device_ms']} ms ({bvh['value']} Mtriangles/s), host {bvh['host_ms']} ms, "
+              f"identical {bvh['identical']}", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(ps, depth)
@@ -350,6 +397,7 @@ def main():
         "cpu_baseline": cpu,
         "adjoint": adjoint,
         "batch": batch,
+        "bvh_build": bvh,
     }
     if rmse:
         out["rmse"] = rmse
