@@ -29,6 +29,7 @@ typedef int32_t zgl_status;
 #define ZGL_INTEGRATOR_NAIVE 0  /* NaivePathIntegrator  (src/integrator/NaivePath.cpp) */
 #define ZGL_INTEGRATOR_LIGHT 1  /* LightPathIntegrator  (src/integrator/LightPath.cpp) */
 #define ZGL_INTEGRATOR_TRIPLE 2 /* TriplePathIntegrator (src/integrator/TriplePath.cpp) */
+#define ZGL_INTEGRATOR_GLOBAL_QUEUE 3 /* GlobalQueuePathIntegrator (src/integrator/GlobalQueuePath.cpp) */
 
 /* ------------------------------------------------------------------ */
 /* Host producers (no GPU needed)                                       */
@@ -158,6 +159,11 @@ typedef struct zgl_pass_params {
     int32_t regen_threshold;      /* naive: lanes of a wave64 still walking the MTBVH when the
                                      wave stops to shade and reissue rays (1..63; 0 = default) */
 } zgl_pass_params;
+/* ZGL_INTEGRATOR_GLOBAL_QUEUE reads first_pass, n_passes, free_counter, max_depth,
+ * russian_roulette, sample_light, light_env_uniform and light_portion; every contribution
+ * is added to the frame as produced (accumulateImage), so the frame must be cleared
+ * (zgl_clear_frame) on reset, as the reference's image clear does.  count_bytes is not
+ * supported for it. */
 
 zgl_status zgl_render(zgl_context* ctx, const zgl_pass_params* params);
 /* Zero the frame (RGBA32F) and the light-splat film (planar R32F, 3W x H). */

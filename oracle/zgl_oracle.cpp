@@ -1405,6 +1405,88 @@ template <class R> struct Ctx {
         return result;
     }
 
+    // ---------------------------------------------------------- global-queue path tracer
+    // GlobalQueuePathIntegrator::renderOnePass (src/integrator/GlobalQueuePath.cpp:100-155)
+    // over one pixel and one pass.  The primary ray (pt_global_queue_primary.glsl:40-86)
+    // takes Sobol dims 0-3; the vertex of depth d = 1..maxDepth
+    // (pt_global_queue_streaming.glsl:89-208) takes dims 4 + 9 d onwards -- NEE selector
+    // and 4 light dims always drawn, then 3 BSDF dims, then the Russian-roulette dim -- and
+    // restarts the pixel's seeds (randSeed, sampleSeed).  Every contribution is added to
+    // the pixel when produced (accumulateImage, :51-55), with no NaN test.
+    void globalQueuePass(int x, int y, int spp, uint32_t fc, R* acc)
+    {
+        auto add = [&](V c) { acc[0] = acc[0] + c.x; acc[1] = acc[1] + c.y; acc[2] = acc[2] + c.z; };
+        seedPixel(x, y, spp, fc);
+        cn.bytes += 64 + 32;
+        R u0 = sample1D(), u1 = sample1D(), u2 = sample1D(), u3 = sample1D();
+        W scr(R(x) / R(s.film_width), R(y) / R(s.film_height));
+        V o, d;
+        cameraRay(scr, u0, u1, u2, u3, o, d);
+        R primDist;
+        int id = bvhHit(o, d, primDist);
+        V pos = o + d * primDist;
+        if (id == -1) { add(envLe(d)); return; }
+        if (id - s.obj_prim_count >= 0) { add(lightLe(id - s.obj_prim_count, pos, -d)); return; }
+        V wo = -d, tp(1);
+        for (int depth = 1; depth <= p.max_depth; depth++) {
+            seedPixel(x, y, spp, fc);
+            dim = 4 + 9 * depth;
+            Surf sf = surface(id, pos);
+            int mat = s.mat_tex_indices[id] & 0xffff;
+            uint32_t t = matType(mat);
+            if (t != 3 && t != 4 && dot(sf.ns, wo) < R(0)) { sf.ns = -sf.ns; sf.ng = -sf.ng; }
+            P mp = loadMat(t, mat);
+            R ud = sample1D();
+            R v0 = sample1D(), v1 = sample1D(), v2 = sample1D(), v3 = sample1D();
+            if (p.sample_light) {
+                Li ls = sampleLightEnv(pos, ud, v0, v1, v2, v3);
+                if (ls.pdf > R(0)) {
+                    V f = evalF(t, mp, wo, ls.wi, sf.ns, 0);
+                    R fp = evalPdf(t, mp, wo, ls.wi, sf.ns);
+                    R wgt = ls.pdf * ls.pdf / (ls.pdf * ls.pdf + fp * fp);
+                    add(f * tp * satDot(sf.ns, ls.wi) * ls.coef * wgt);
+                }
+            }
+            R a0 = sample1D(), a1 = sample1D(), a2 = sample1D();
+            S bs = sampleF(t, mp, sf.ns, wo, 0, V(a0, a1, a2));
+            bool delta = bs.flag == (1u << 3) || bs.flag == (1u << 4);
+            if (bs.pdf < R(1e-8f)) return;
+            tp = tp * (bs.f / bs.pdf * (delta ? R(1) : absDot(sf.ns, bs.wi)));
+            V no = pos + bs.wi * R(1e-4f);
+            R dist;
+            int next = bvhHit(no, bs.wi, dist);
+            int lid = next - s.obj_prim_count;
+            V np = no + bs.wi * dist;
+            if (next == -1) {
+                V rad = envLe(bs.wi);
+                R wgt = R(1);
+                if (p.sample_light && !delta) {
+                    R ep = envPdfLi(bs.wi) * pdfSelEnv();
+                    wgt = (ep <= R(0)) ? R(0) : bs.pdf * bs.pdf / (bs.pdf * bs.pdf + ep * ep);
+                }
+                add(rad * tp * wgt);
+                return;
+            } else if (lid >= 0) {
+                V rad = lightLe(lid, np, -bs.wi);
+                R wgt = R(1);
+                if (p.sample_light && !delta) {
+                    R lp = lightPdfLi(lid, pos, np) * pdfSelLight(lid);
+                    wgt = (lp <= R(0)) ? R(0) : bs.pdf * bs.pdf / (bs.pdf * bs.pdf + lp * lp);
+                }
+                add(rad * tp * wgt);
+                return;
+            }
+            if (p.russian_roulette) {
+                R cp = gmin(maxComp(bs.f / bs.pdf), R(0.95f));
+                if (sample1D() >= cp) return;
+                tp = tp / cp;
+            }
+            id = next;
+            pos = np;
+            wo = -bs.wi;
+        }
+    }
+
     // ---------------------------------------------------------- differentiable naive
     // The naive estimator with sampling at the scene (theta0) and BSDF values of
     // differentiable materials (principled, Lambertian) and environment radiance at
@@ -1882,6 +1964,10 @@ static void renderRegion(const orc_scene& s, const orc_params& p, int x0, int y0
             for (int k = 0; k < p.n_passes; k++) {
                 const int spp = p.first_pass + k;
                 Ctx<R> c(s, p, cn, samplerType);
+                if (p.integrator == 3) {
+                    c.globalQueuePass(x, y, spp, (uint32_t)(p.free_counter + k), acc);
+                    continue;
+                }
                 c.seedPixel(x, y, spp, (uint32_t)(p.free_counter + k));
                 cn.bytes += 64 + 32;
                 R u0 = c.sample1D(), u1 = c.sample1D(), u2 = c.sample1D(), u3 = c.sample1D();
@@ -2253,7 +2339,8 @@ int orc_trace_closest(const orc_scene* s, const float* rays, int64_t n, int32_t*
     return 0;
 }
 
-// Camera-path integrators (naive = 0, triple camera pass = 2) over a pixel region:
+// Camera-path integrators (naive = 0, triple camera pass = 2, global queue = 3) over a
+// pixel region:
 // out[h][w][3] = per-pixel sums over the passes.  fp64 selects the double instance.
 int orc_render_pixels(const orc_scene* s, const orc_params* p, int x0, int y0, int w, int h, float* out,
                       uint64_t* counters, int nthreads, int fp64)

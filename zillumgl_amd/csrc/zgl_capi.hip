@@ -75,6 +75,8 @@ struct zgl_context {
     // differentiable mode: evaluation point, adjoint input and outputs
     DevBuf evalMat, evalEnv, dFrame, dMat, dEnv;
     DevBuf bvhBounds;  // node bounds of a device-built MTBVH
+    DevBuf gqQueues, gqCounts;  // global-queue path tracer
+    int gqBlocksPerCU = 0;
     DeviceBvh bvh;     // work memory of the device MTBVH build
     std::vector<float> hostMaterials;  // uploaded material records (16 floats each)
     int numCUs = 0;
@@ -90,7 +92,7 @@ struct zgl_context {
         DevBuf* all[] = {&rows, &tris, &indices, &vertices, &normals, &texcoords, &matTex, &materials,
                          &lightPower, &lightAlias, &lightProb, &env, &envAlias, &envProb, &noise, &sobolM,
                          &sobolTab, &frame, &film, &counters, &scratchRays, &scratchOut, &workCounter,
-                         &evalMat, &evalEnv, &dFrame, &dMat, &dEnv, &bvhBounds};
+                         &evalMat, &evalEnv, &dFrame, &dMat, &dEnv, &bvhBounds, &gqQueues, &gqCounts};
         for (DevBuf* b : all) b->release();
         bvh.release();
     }
@@ -549,12 +551,15 @@ static void preparePasses(zgl_context* ctx, const zgl_pass_params* p, DevScene& 
     if (p->n_passes < 1 || p->n_passes > 65536 || p->first_pass < 0 || p->first_pass > (1 << 24) ||
         p->max_depth < 0 || p->max_depth > 1024)
         invalid("pass range / depth");
-    if (p->integrator < 0 || p->integrator > 2) invalid("integrator");
+    if (p->integrator < 0 || p->integrator > 3) invalid("integrator");
+    if (p->integrator == ZGL_INTEGRATOR_GLOBAL_QUEUE && p->count_bytes)
+        invalid("count_bytes is not supported by the global-queue integrator");
     if ((p->integrator == ZGL_INTEGRATOR_LIGHT || p->integrator == ZGL_INTEGRATOR_TRIPLE) &&
         ctx->scene.nLightTriangles < 1)
         invalid("light and triple integrators need light triangles");
     if (p->regen_threshold < 0 || p->regen_threshold > 63) invalid("regen_threshold must be in [0, 63]");
-    if (p->integrator != ZGL_INTEGRATOR_NAIVE && (p->blocks_one_pass < 1 || p->blocks_one_pass > 65536))
+    if ((p->integrator == ZGL_INTEGRATOR_LIGHT || p->integrator == ZGL_INTEGRATOR_TRIPLE) &&
+        (p->blocks_one_pass < 1 || p->blocks_one_pass > 65536))
         invalid("blocks_one_pass");
     if (p->integrator == ZGL_INTEGRATOR_TRIPLE && (p->loops_per_pass < 1 || p->loops_per_pass > 1024))
         invalid("loops_per_pass");
@@ -596,6 +601,13 @@ zgl_status zgl_render(zgl_context* ctx, const zgl_pass_params* p)
             hipCheck(launchNaivePersistent(s, pp, frame, (int*)ctx->workCounter.p, threshold,
                                            ctx->numCUs * ctx->naiveBlocksPerCU, cnt, ctx->stream),
                      "naive pass");
+        } else if (p->integrator == ZGL_INTEGRATOR_GLOBAL_QUEUE) {
+            if (!ctx->gqBlocksPerCU) ctx->gqBlocksPerCU = globalQueueBlocks();
+            ctx->ensure(ctx->gqQueues, globalQueueBytes(ctx->filmW, ctx->filmH), "global queue");
+            ctx->ensure(ctx->gqCounts, (size_t)(pp.maxDepth + 1) * 4, "global queue counts");
+            hipCheck(launchGlobalQueue(s, pp, frame, ctx->gqQueues.p, (int*)ctx->gqCounts.p,
+                                       ctx->numCUs * ctx->gqBlocksPerCU, ctx->stream),
+                     "global-queue pass");
         } else if (p->integrator == ZGL_INTEGRATOR_LIGHT) {
             s.sampler = 0;  // LightPath.cpp:49 forces the independent sampler
             hipCheck(launchLight(s, pp, film, cnt, ctx->stream), "light pass");

@@ -57,6 +57,13 @@ hipError_t launchNaiveBatch(const BatchScene* items, int nScenes, int* sceneCoun
 hipError_t launchNaiveReplay(const DevScene& s, const PassParams& p, const float4* evalMat, const float4* evalEnv,
                              float4* frame, const float4* dFrame, double* dMat, double* dEnv, int envCopies,
                              double* envSum, hipStream_t st);
+// Global-queue path tracer: device bytes of the two item queues for a w x h film, blocks
+// per CU of the extension kernel, and the launch of p.nPasses passes (counts: maxDepth + 1
+// ints of device memory).
+size_t globalQueueBytes(int w, int h);
+int globalQueueBlocks();
+hipError_t launchGlobalQueue(const DevScene& s, const PassParams& p, float4* frame, void* queues, int* counts,
+                             int extendBlocks, hipStream_t st);
 hipError_t launchEnvExpand(const ushort4* env, float4* out, int64_t n, hipStream_t st);
 hipError_t launchLight(const DevScene& s, const PassParams& p, float* film, unsigned long long* counters,
                        hipStream_t st);

@@ -279,6 +279,48 @@ class NaivePathIntegrator:
         return 1.0 / (self.cur_sample + 1)
 
 
+class GlobalQueuePathIntegrator:
+    """Global-queue (wavefront) path tracer (src/integrator/GlobalQueuePath.cpp:100-176);
+    QueuedPTParam defaults (src/core/Integrator.h:211-222).  Its estimator differs from
+    NaivePathIntegrator's (Sobol dims 4 + 9 depth per vertex, seeds restarted per depth,
+    contributions added to the frame as produced)."""
+
+    def __init__(self, max_depth=4, russian_roulette=False, sample_light=True, light_env_uniform=False,
+                 light_portion=0.5):
+        self.max_depth = max_depth
+        self.russian_roulette = russian_roulette
+        self.sample_light = sample_light
+        self.light_env_uniform = light_env_uniform
+        self.light_portion = light_portion
+        self.cur_sample = 0
+        self.free_counter = 0
+        self.sample_per_pixel = 0.0
+
+    def init(self, ctx):
+        self.ctx = ctx
+        self.reset()
+
+    def reset(self):
+        self.ctx.clear()
+        self.cur_sample = 0
+        self.sample_per_pixel = 1.0
+
+    def render_one_pass(self, n_passes=1):
+        """renderOnePass; n_passes > 1 renders consecutive passes in one call."""
+        self.free_counter += 1
+        self.ctx.render(_lib.INTEGRATOR_GLOBAL_QUEUE, self.cur_sample, n_passes, self.free_counter, self.max_depth,
+                        self.russian_roulette, self.sample_light, self.light_env_uniform, self.light_portion)
+        self.free_counter += n_passes - 1
+        self.cur_sample += n_passes
+        self.sample_per_pixel += n_passes
+
+    def get_frame(self):
+        return self.ctx.read_frame()
+
+    def result_scale(self):
+        return 1.0 / self.sample_per_pixel
+
+
 class LightPathIntegrator:
     """Light tracer (src/integrator/LightPath.cpp:83-135); LightPathIntegParam defaults."""
     WORKGROUP = 1536
