@@ -14,6 +14,9 @@
 
 using namespace zgl;
 
+// Lanes still walking when a wave of the persistent schedule stops to shade.
+constexpr int kDefaultRegenThreshold = 10;
+
 namespace {
 
 struct DevBuf {
@@ -108,7 +111,7 @@ struct zgl_batch {
     std::vector<int32_t> order;    // launch position -> index in ctxs
     DevBuf items, counters, sobolTab;
     int numCUs = 0, maxBlocksPerCU = 1;
-    int threshold = 16, blocksPerCU = 0;
+    int threshold = kDefaultRegenThreshold, blocksPerCU = 0;
     int sobolFirst = 0, sobolCount = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -161,8 +164,6 @@ template <class F> zgl_status guardedHost(F&& f)
 
 }  // namespace
 
-// Lanes still walking when a wave of the persistent schedule stops to shade.
-constexpr int kDefaultRegenThreshold = 16;
 
 extern "C" {
 

@@ -341,9 +341,11 @@ __device__ __forceinline__ void naiveBody(const DevScene& s0, const PassParams& 
         // that reaches a leaf whose box is hit waits there, and the waiting lanes test
         // their triangles together once per round, so that the triangle test (as long
         // as the box test) runs once per round instead of on nearly every row step for
-        // a few lanes.
+        // a few lanes.  A lane whose shadow walk ends settles its NEE term and starts its
+        // extension walk within the loop (the same operations, in the same order, as the
+        // shading step would do), instead of idling until the wave leaves the loop.
         {
-            const bool general = __ballot(walking && wk.p.special) == 0ull;
+            bool general = __ballot(walking && wk.p.special) == 0ull;
             for (;;) {
 #pragma unroll
                 for (int u = 0; u < kRowsPerRound; ++u) {
@@ -354,6 +356,21 @@ __device__ __forceinline__ void naiveBody(const DevScene& s0, const PassParams& 
                 if (walking && wk.leaf >= 0) {
                     if (walkLeaf<C>(s, wk, cn)) walking = false;
                 }
+                // a shadow walk that ended settles its NEE term and starts the extension walk
+                if (!walking && phase == kShadow) {
+                    if (wk.closest < 0) {
+                        if (C) cn.bytes += pendBytes;
+                        if (pendValid) result += pend;
+                    }
+                    if (terminate) {
+                        phase = kFinish;
+                    } else {
+                        walkStart<C>(s, wk, rayOffseted(pos, wi), false, 1e8f, cn);
+                        walking = true;
+                        phase = kExtend;
+                    }
+                }
+                general = __ballot(walking && wk.p.special) == 0ull;
                 if (__popcll(__ballot(walking)) <= threshold) break;
             }
         }
