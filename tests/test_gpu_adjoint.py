@@ -116,6 +116,24 @@ def test_adjoint_matches_oracle_fp64(ctx, c2s):
     assert em <= 5e-3 and ee <= 1e-3
 
 
+def test_adjoint_deep_paths_match_oracle_fp64(ctx, c2s):
+    """Depth 10 issues up to 21 traversal queries per sample, more than the replay's hit
+    log holds (16): the queries past it are walked again in the replay pass."""
+    print("[gpu] adjoint: depth 10 vs oracle FP64 adjoint", flush=True)
+    ctx.upload(c2s)
+    rng = np.random.default_rng(4)
+    _, _, p1, e1 = _perturb(c2s, 3)
+    d_frame = rng.uniform(-1, 1, (c2s.height, c2s.width, 3)).astype(np.float32)
+    ctx.set_eval_params(p1, e1)
+    gm, ge = ctx.adjoint(d_frame, p1.shape[0], e1.shape, 0, 2, 1, 10)
+    rm, re = orc.adjoint(c2s, orc.params(0, 0, 2, 1, 10), _records(c2s, p1), e1, d_frame, fp32_values=True)
+    em, nm = _close(gm, rm, 1e-3)
+    ee, ne = _close(ge, re, 1e-3)
+    print(f"[gpu] adjoint depth 10: material max rel err {em:.3g} over {nm}, env {ee:.3g} over {ne}", flush=True)
+    assert nm >= 20 and ne >= 1000
+    assert em <= 5e-3 and ee <= 1e-3
+
+
 def test_autograd_l2_gradient_matches_central_differences(ctx, c2s):
     print("[gpu] adjoint: torch autograd L2 loss vs central differences", flush=True)
     ctx.upload(c2s)

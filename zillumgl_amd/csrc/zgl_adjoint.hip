@@ -135,9 +135,45 @@ ZGL_D void acc(Sum3& a, vec3 t, double sign)
     a.z += sign * (double)t.z;
 }
 
+// Traversal results of one sample.  The first pass records every closest-hit query
+// (primitive, distance) and every shadow query (occluded) in order; the replay pass,
+// which issues the same queries with the same rays, reads them back instead of walking
+// the MTBVH again.  Queries past the log's capacity (paths deeper than the log holds)
+// are walked again in the replay, which gives the same results.
+constexpr int kHitLog = 16;
+struct HitLog {
+    int n;
+    int id[kHitLog];
+    float dist[kHitLog];
+};
+template <bool ADJ> ZGL_D int loggedHit(const DevScene& s, const Ray& r, float& dist, HitLog& log)
+{
+    const int i = log.n++;
+    if (ADJ && i < kHitLog) {
+        dist = log.dist[i];
+        return log.id[i];
+    }
+    Counters cn;
+    const int id = bvhHit<false>(s, r, dist, cn);
+    if (!ADJ && i < kHitLog) {
+        log.id[i] = id;
+        log.dist[i] = dist;
+    }
+    return id;
+}
+template <bool ADJ> ZGL_D bool loggedTest(const DevScene& s, const Ray& r, float tmax, HitLog& log)
+{
+    const int i = log.n++;
+    if (ADJ && i < kHitLog) return log.id[i] != 0;
+    Counters cn;
+    const bool occluded = bvhTest<false>(s, r, tmax, cn);
+    if (!ADJ && i < kHitLog) log.id[i] = occluded ? 1 : 0;
+    return occluded;
+}
+
 template <bool ADJ>
 ZGL_D vec3 replayPath(const DevScene& s, const Replay& rp, const PassParams& prm, vec2 scr, Rng& rng, Sum3& rest,
-                      vec3 delta, const Grads& gr)
+                      vec3 delta, const Grads& gr, HitLog& log)
 {
     // first pass: rest accumulates the terms; replay: rest loses them as they are met
     const double sign = ADJ ? -1.0 : 1.0;
@@ -150,7 +186,8 @@ ZGL_D vec3 replayPath(const DevScene& s, const Replay& rp, const PassParams& prm
     const vec4 u = sample4D<false>(s, rng, cn);
     const Ray ray = cameraSampleRay(s, scr, u);
     float primDist;
-    int id = bvhHit<false>(s, ray, primDist, cn);
+    log.n = 0;
+    int id = loggedHit<ADJ>(s, ray, primDist, log);
     vec3 pos = rayPoint(ray, primDist);
     if (id == -1) {
         EnvTaps tap;
@@ -195,7 +232,7 @@ ZGL_D vec3 replayPath(const DevScene& s, const Replay& rp, const PassParams& prm
                     const float dist = distance(pos, y);
                     float pdf = dist * dist / (triangleArea(a, b, c) * cosTheta);
                     const float test = dist - 1e-4f - 1e-6f;
-                    if (!bvhTest<false>(s, rayOffseted(pos, wl), test, cn) && !(pdf < 1e-8f)) {
+                    if (!loggedTest<ADJ>(s, rayOffseted(pos, wl), test, log) && !(pdf < 1e-8f)) {
                         const vec3 weight = lightLe<false>(s, lid, y, -wl, cn);
                         pdf *= lightPdfSampleOne<false>(s, lid, cn);
                         ls = LightLiSample{wl, weight / pdf, pdf};
@@ -205,7 +242,7 @@ ZGL_D vec3 replayPath(const DevScene& s, const Replay& rp, const PassParams& prm
             } else {
                 const vec4 sp = envSampleWi<false>(s, us, cn);
                 const vec3 we = v3(sp.x, sp.y, sp.z);
-                if (!bvhTest<false>(s, rayOffseted(pos, we), 1e8f, cn) && sp.w != 0.0f) {
+                if (!loggedTest<ADJ>(s, rayOffseted(pos, we), 1e8f, log) && sp.w != 0.0f) {
                     envPdf = sp.w;
                     ls = LightLiSample{we, envLeEval(s, rp.evalEnv, we, tap) / sp.w, sp.w};
                     valid = true;
@@ -251,7 +288,7 @@ ZGL_D vec3 replayPath(const DevScene& s, const Replay& rp, const PassParams& prm
 
         float dist;
         const Ray next = rayOffseted(pos, bs.wi);
-        const int nextId = bvhHit<false>(s, next, dist, cn);
+        const int nextId = loggedHit<ADJ>(s, next, dist, log);
         const int lightId = nextId - s.objPrimCount;
         const vec3 nextPos = rayPoint(next, dist);
         if (nextId == -1) {
@@ -298,14 +335,14 @@ ZGL_D vec3 replayPath(const DevScene& s, const Replay& rp, const PassParams& prm
 template <int MODE>
 __global__ __launch_bounds__(256) void k_naive_replay(DevScene s, PassParams prm, Replay rp, float4* __restrict__ frame,
                                                       const float4* __restrict__ dFrame, double* __restrict__ dMat,
-                                                      double* __restrict__ dEnv, int envCopies)
+                                                      double* __restrict__ dEnv, int envCopies, int passChunk)
 {
     const bool useLds = MODE == 1 && s.nMaterials <= kLdsMaterials;
     if (useLds) {
         for (int i = threadIdx.x; i < kLdsMaterials * NumMatGrad; i += blockDim.x) sMatGrad[i] = 0.0;
         __syncthreads();
     }
-    const int copy = (int)((blockIdx.y * gridDim.x + blockIdx.x) % (unsigned)envCopies);
+    const int copy = (int)(((blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x) % (unsigned)envCopies);
     const Grads gr{useLds, dMat, dEnv + (size_t)copy * ((size_t)s.envW * s.envH * 3)};
     const int x = blockIdx.x * 16 + (threadIdx.x & 15);
     const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
@@ -320,7 +357,12 @@ __global__ __launch_bounds__(256) void k_naive_replay(DevScene s, PassParams prm
             const float4 d = dFrame[pix];
             delta = v3(d.x, d.y, d.z);
         }
-        for (int k = 0; k < prm.nPasses; k++) {
+        // MODE 1: blockIdx.z selects a chunk of the passes (the gradients are sums over
+        // samples, in any order); MODE 0 keeps a pixel's passes in one thread, in order
+        const int k0 = MODE == 1 ? (int)blockIdx.z * passChunk : 0;
+        const int k1 = MODE == 1 ? min(prm.nPasses, k0 + passChunk) : prm.nPasses;
+        HitLog log;
+        for (int k = k0; k < k1; k++) {
             const int spp = prm.firstPass + k;
             const uint32_t fc = (uint32_t)(prm.freeCounter + k);
             Rng rng;
@@ -330,7 +372,7 @@ __global__ __launch_bounds__(256) void k_naive_replay(DevScene s, PassParams prm
             rng.dim = 0;
             const Rng start = rng;
             Sum3 rest = {0.0, 0.0, 0.0};
-            const vec3 c = replayPath<false>(s, rp, prm, scr, rng, rest, v3(0.0f), gr);
+            const vec3 c = replayPath<false>(s, rp, prm, scr, rng, rest, v3(0.0f), gr, log);
             if (hasNan(c)) continue;
             if (MODE == 0) {
                 const vec3 last = (spp == 0) ? v3(0.0f) : v3(mem.x, mem.y, mem.z);
@@ -339,7 +381,7 @@ __global__ __launch_bounds__(256) void k_naive_replay(DevScene s, PassParams prm
                 stored = true;
             } else if (delta.x != 0.0f || delta.y != 0.0f || delta.z != 0.0f) {
                 rng = start;
-                replayPath<true>(s, rp, prm, scr, rng, rest, delta, gr);
+                replayPath<true>(s, rp, prm, scr, rng, rest, delta, gr, log);
             }
         }
         if (MODE == 0 && stored) frame[pix] = mem;
@@ -380,6 +422,8 @@ __global__ void k_sum_copies(const double* __restrict__ src, double* __restrict_
     dst[i] = v;
 }
 
+constexpr int kPassChunk = 8;
+
 hipError_t launchNaiveReplay(const DevScene& s, const PassParams& p, const float4* evalMat, const float4* evalEnv,
                              float4* frame, const float4* dFrame, double* dMat, double* dEnv, int envCopies,
                              double* envSum, hipStream_t st)
@@ -387,11 +431,16 @@ hipError_t launchNaiveReplay(const DevScene& s, const PassParams& p, const float
     const Replay rp{evalMat, evalEnv};
     const dim3 grid((s.filmW + 15) / 16, (s.filmH + 15) / 16);
     if (!dFrame) {
-        hipLaunchKernelGGL(k_naive_replay<0>, grid, dim3(256), 0, st, s, p, rp, frame, dFrame, dMat, dEnv, 1);
+        hipLaunchKernelGGL(k_naive_replay<0>, grid, dim3(256), 0, st, s, p, rp, frame, dFrame, dMat, dEnv, 1,
+                           p.nPasses);
         return hipGetLastError();
     }
     if (envCopies < 1) envCopies = 1;
-    hipLaunchKernelGGL(k_naive_replay<1>, grid, dim3(256), 0, st, s, p, rp, frame, dFrame, dMat, dEnv, envCopies);
+    // passes in chunks of kPassChunk: more, shorter threads balance the waves
+    const int chunks = (p.nPasses + kPassChunk - 1) / kPassChunk;
+    const dim3 gridAdj(grid.x, grid.y, (unsigned)chunks);
+    hipLaunchKernelGGL(k_naive_replay<1>, gridAdj, dim3(256), 0, st, s, p, rp, frame, dFrame, dMat, dEnv, envCopies,
+                       kPassChunk);
     const int64_t n = (int64_t)s.envW * s.envH * 3;
     hipLaunchKernelGGL(k_sum_copies, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, dEnv, envSum, n,
                        envCopies);
