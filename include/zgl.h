@@ -30,6 +30,7 @@ typedef int32_t zgl_status;
 #define ZGL_INTEGRATOR_LIGHT 1  /* LightPathIntegrator  (src/integrator/LightPath.cpp) */
 #define ZGL_INTEGRATOR_TRIPLE 2 /* TriplePathIntegrator (src/integrator/TriplePath.cpp) */
 #define ZGL_INTEGRATOR_GLOBAL_QUEUE 3 /* GlobalQueuePathIntegrator (src/integrator/GlobalQueuePath.cpp) */
+#define ZGL_INTEGRATOR_NAIVE_WAVEFRONT 4 /* NaivePathIntegrator's estimator and frame, wavefront schedule */
 
 /* ------------------------------------------------------------------ */
 /* Host producers (no GPU needed)                                       */

@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(HERE, "libzgl.so")
 
 ZGL_OK, ZGL_E_INVALID, ZGL_E_HIP, ZGL_E_OOM, ZGL_E_STATE, ZGL_E_NODEVICE = range(6)
 INTEGRATOR_NAIVE, INTEGRATOR_LIGHT, INTEGRATOR_TRIPLE, INTEGRATOR_GLOBAL_QUEUE = 0, 1, 2, 3
+INTEGRATOR_NAIVE_WAVEFRONT = 4
 STATUS_NAMES = {0: "ZGL_OK", 1: "ZGL_E_INVALID", 2: "ZGL_E_HIP", 3: "ZGL_E_OOM", 4: "ZGL_E_STATE",
                 5: "ZGL_E_NODEVICE"}
 

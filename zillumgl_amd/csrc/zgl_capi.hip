@@ -80,6 +80,8 @@ struct zgl_context {
     DevBuf bvhBounds;  // node bounds of a device-built MTBVH
     DevBuf gqQueues, gqCounts;  // global-queue path tracer
     int gqBlocksPerCU = 0;
+    DevBuf wfWork;  // wavefront schedule of the naive tracer
+    int wfTraceBlocksPerCU = 0, wfShadeBlocksPerCU = 0;
     DeviceBvh bvh;     // work memory of the device MTBVH build
     std::vector<float> hostMaterials;  // uploaded material records (16 floats each)
     int numCUs = 0;
@@ -95,7 +97,7 @@ struct zgl_context {
         DevBuf* all[] = {&rows, &tris, &indices, &vertices, &normals, &texcoords, &matTex, &materials,
                          &lightPower, &lightAlias, &lightProb, &env, &envAlias, &envProb, &noise, &sobolM,
                          &sobolTab, &frame, &film, &counters, &scratchRays, &scratchOut, &workCounter,
-                         &evalMat, &evalEnv, &dFrame, &dMat, &dEnv, &bvhBounds, &gqQueues, &gqCounts};
+                         &evalMat, &evalEnv, &dFrame, &dMat, &dEnv, &bvhBounds, &gqQueues, &gqCounts, &wfWork};
         for (DevBuf* b : all) b->release();
         bvh.release();
     }
@@ -552,7 +554,8 @@ static void preparePasses(zgl_context* ctx, const zgl_pass_params* p, DevScene& 
     if (p->n_passes < 1 || p->n_passes > 65536 || p->first_pass < 0 || p->first_pass > (1 << 24) ||
         p->max_depth < 0 || p->max_depth > 1024)
         invalid("pass range / depth");
-    if (p->integrator < 0 || p->integrator > 3) invalid("integrator");
+    if (p->integrator < 0 || p->integrator > 4) invalid("integrator");
+    if (p->integrator == 4 && p->count_bytes) invalid("count_bytes is not supported by the wavefront schedule");
     if (p->integrator == ZGL_INTEGRATOR_GLOBAL_QUEUE && p->count_bytes)
         invalid("count_bytes is not supported by the global-queue integrator");
     if ((p->integrator == ZGL_INTEGRATOR_LIGHT || p->integrator == ZGL_INTEGRATOR_TRIPLE) &&
@@ -602,6 +605,15 @@ zgl_status zgl_render(zgl_context* ctx, const zgl_pass_params* p)
             hipCheck(launchNaivePersistent(s, pp, frame, (int*)ctx->workCounter.p, threshold,
                                            ctx->numCUs * ctx->naiveBlocksPerCU, cnt, ctx->stream),
                      "naive pass");
+        } else if (p->integrator == ZGL_INTEGRATOR_NAIVE_WAVEFRONT) {
+            if (!ctx->wfTraceBlocksPerCU) wavefrontBlocks(ctx->wfTraceBlocksPerCU, ctx->wfShadeBlocksPerCU);
+            // up to 24 GB of path state: 44 passes of 1080p at once
+            const int batch = wavefrontBatch(ctx->filmW, ctx->filmH, pp.nPasses, (size_t)24 << 30);
+            ctx->ensure(ctx->wfWork, wavefrontBytes(ctx->filmW, ctx->filmH, batch) + (size_t)3 * (pp.maxDepth + 2) * 4,
+                        "wavefront work");
+            hipCheck(launchNaiveWavefront(s, pp, frame, ctx->wfWork.p, batch, ctx->numCUs * ctx->wfTraceBlocksPerCU,
+                                          ctx->numCUs * ctx->wfShadeBlocksPerCU, ctx->stream),
+                     "wavefront pass");
         } else if (p->integrator == ZGL_INTEGRATOR_GLOBAL_QUEUE) {
             if (!ctx->gqBlocksPerCU) ctx->gqBlocksPerCU = globalQueueBlocks();
             ctx->ensure(ctx->gqQueues, globalQueueBytes(ctx->filmW, ctx->filmH), "global queue");

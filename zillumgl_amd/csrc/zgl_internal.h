@@ -64,6 +64,15 @@ size_t globalQueueBytes(int w, int h);
 int globalQueueBlocks();
 hipError_t launchGlobalQueue(const DevScene& s, const PassParams& p, float4* frame, void* queues, int* counts,
                              int extendBlocks, hipStream_t st);
+// Naive path tracer on the wavefront schedule (same frame as launchNaivePersistent):
+// device bytes of its work buffer for a w x h film, blocks per CU of its trace and shade
+// kernels, and the launch.
+size_t wavefrontBytes(int w, int h, int passes);
+// Passes rendered together (memory budget in bytes), spread evenly over the batches.
+int wavefrontBatch(int w, int h, int nPasses, size_t budget);
+void wavefrontBlocks(int& traceBlocksPerCU, int& shadeBlocksPerCU);
+hipError_t launchNaiveWavefront(const DevScene& s, const PassParams& p, float4* frame, void* work, int batch,
+                                int traceBlocks, int shadeBlocks, hipStream_t st);
 hipError_t launchEnvExpand(const ushort4* env, float4* out, int64_t n, hipStream_t st);
 hipError_t launchLight(const DevScene& s, const PassParams& p, float* film, unsigned long long* counters,
                        hipStream_t st);
