@@ -35,6 +35,7 @@ enum : int {
 // MTBVH rows each walking lane advances per round of the walking loop (one leaf test
 // and one ballot per round).
 constexpr int kRowsPerRound = 4;
+constexpr int kRegions = 8;  // pixel regions, one per XCD
 
 // Pixel order for the dynamic fetch: 8x8 tiles in row-major tile order.
 __device__ __forceinline__ void pixelOfIndex(int idx, int W, int& x, int& y)
@@ -47,7 +48,7 @@ __device__ __forceinline__ void pixelOfIndex(int idx, int W, int& x, int& y)
 
 }  // namespace
 
-__global__ void k_reset_counter(int* c) { if (threadIdx.x == 0 && blockIdx.x == 0) c[0] = 0; }
+__global__ void k_reset_counter(int* c) { if (threadIdx.x < kRegions && blockIdx.x == 0) c[threadIdx.x] = 0; }
 
 template <bool C> __device__ void flushCountersP(const Counters& cn, unsigned long long* out)
 {
@@ -69,7 +70,9 @@ __device__ __forceinline__ void naiveBody(const DevScene& s0, const PassParams& 
 {
     const int lane = threadIdx.x & 63;
     Counters cn;
-    int cs = 0;  // batch: the wave's current scene (the same in every lane)
+    int cs = 0;
+    int region = (int)(blockIdx.x % kRegions);  // the workgroup's XCD under round-robin dispatch
+    int exhausted = 0;  // batch: the wave's current scene (the same in every lane)
 
     int phase = kNewPixel;
     int pix = -1, pass = 0;
@@ -281,22 +284,43 @@ __device__ __forceinline__ void naiveBody(const DevScene& s0, const PassParams& 
             }
         }
 
-        // pixels are handed out per wave: one atomic for all lanes that need one
+        // pixels are handed out per wave: one atomic for all lanes that need one.  One
+        // region of 8x8 tiles per XCD (workgroups are dispatched to XCDs round-robin), so
+        // that the waves sharing an L2 start with neighbouring pixels; a wave whose region
+        // is used up moves on to the next, and stops once all eight are.
         const unsigned long long need = __ballot(!walking && phase == kNewPixel);
         if (need) {
             if (!walking && phase == kNewPixel && pix >= 0 && stored) frame[pix] = mem;
             const int n = __popcll(need);
             const int leader = __ffsll((long long)need) - 1;
             int base = 0;
-            if (lane == leader) base = atomicAdd(pixelCounter, n);
-            base = __shfl(base, leader);
+            int limit = nIndex;
+            if (BATCH) {
+                if (lane == leader) base = atomicAdd(pixelCounter, n);
+                base = __shfl(base, leader);
+            } else {
+                // the wave's region of the image first (one per XCD), then the others
+                const int len = ((nIndex / kRegions + 63) / 64) * 64;
+                const int start = min(nIndex, region * len);
+                limit = min(nIndex, start + len);
+                if (lane == leader) base = atomicAdd(pixelCounter + region, n);
+                base = __shfl(base, leader) + start;
+                if (base >= limit) {
+                    region = (region + 1) % kRegions;
+                    exhausted++;
+                } else {
+                    exhausted = 0;
+                }
+            }
             if (!walking && phase == kNewPixel) {
                 const unsigned long long below = need & ((1ull << lane) - 1ull);
                 const int idx = base + __popcll(below);
                 pix = -1;
                 stored = false;
-                if (idx >= nIndex) {
+                if (BATCH ? idx >= nIndex : exhausted >= kRegions) {
                     phase = BATCH ? kWait : kDone;
+                } else if (!BATCH && idx >= limit) {
+                    // this region is used up: fetch again from the next one
                 } else {
                     int x, y;
                     pixelOfIndex(idx, W, x, y);
