@@ -80,7 +80,8 @@ struct zgl_context {
     DevBuf bvhBounds;  // node bounds of a device-built MTBVH
     DevBuf gqQueues, gqCounts;  // global-queue path tracer
     int gqBlocksPerCU = 0;
-    DevBuf wfWork;  // wavefront schedule of the naive tracer
+    DevBuf wfWork;    // wavefront schedule of the naive tracer
+    DevBuf naiveRes;  // per-sample values of the persistent naive tracer
     int wfTraceBlocksPerCU = 0, wfShadeBlocksPerCU = 0;
     DeviceBvh bvh;     // work memory of the device MTBVH build
     std::vector<float> hostMaterials;  // uploaded material records (16 floats each)
@@ -97,7 +98,8 @@ struct zgl_context {
         DevBuf* all[] = {&rows, &tris, &indices, &vertices, &normals, &texcoords, &matTex, &materials,
                          &lightPower, &lightAlias, &lightProb, &env, &envAlias, &envProb, &noise, &sobolM,
                          &sobolTab, &frame, &film, &counters, &scratchRays, &scratchOut, &workCounter,
-                         &evalMat, &evalEnv, &dFrame, &dMat, &dEnv, &bvhBounds, &gqQueues, &gqCounts, &wfWork};
+                         &evalMat, &evalEnv, &dFrame, &dMat, &dEnv, &bvhBounds, &gqQueues, &gqCounts, &wfWork,
+                         &naiveRes};
         for (DevBuf* b : all) b->release();
         bvh.release();
     }
@@ -602,8 +604,12 @@ zgl_status zgl_render(zgl_context* ctx, const zgl_pass_params* p)
         if (p->integrator == ZGL_INTEGRATOR_NAIVE) {
             if (!ctx->naiveBlocksPerCU) ctx->naiveBlocksPerCU = naivePersistentBlocksPerCU();
             const int threshold = p->regen_threshold > 0 ? p->regen_threshold : kDefaultRegenThreshold;
-            hipCheck(launchNaivePersistent(s, pp, frame, (int*)ctx->workCounter.p, threshold,
-                                           ctx->numCUs * ctx->naiveBlocksPerCU, cnt, ctx->stream),
+            // per-sample values of up to 4 GB worth of passes (121 passes at 1080p)
+            const size_t perPass = (size_t)ctx->filmW * ctx->filmH * 16;
+            const int resPasses = (int)std::max<size_t>(1, std::min<size_t>(pp.nPasses, ((size_t)4 << 30) / perPass));
+            ctx->ensure(ctx->naiveRes, perPass * resPasses, "sample values");
+            hipCheck(launchNaivePersistent(s, pp, frame, (float4*)ctx->naiveRes.p, resPasses, (int*)ctx->workCounter.p,
+                                           threshold, ctx->numCUs * ctx->naiveBlocksPerCU, cnt, ctx->stream),
                      "naive pass");
         } else if (p->integrator == ZGL_INTEGRATOR_NAIVE_WAVEFRONT) {
             if (!ctx->wfTraceBlocksPerCU) wavefrontBlocks(ctx->wfTraceBlocksPerCU, ctx->wfShadeBlocksPerCU);

@@ -45,8 +45,11 @@ hipError_t launchTraceClosest(const DevScene& s, const float* rays, int64_t n, i
                               uint32_t* bin, unsigned long long* counters, hipStream_t st);
 hipError_t launchTraceAny(const DevScene& s, const float* rays, const float* tmax, int64_t n,
                           uint8_t* occluded, hipStream_t st);
-hipError_t launchNaivePersistent(const DevScene& s, const PassParams& p, float4* frame, int* pixelCounter,
-                                 int threshold, int blocks, unsigned long long* counters, hipStream_t st);
+// res: per-sample values of up to resPasses passes (resPasses * W * H float4); the
+// launch runs the passes in chunks of resPasses and adds each chunk to the frame in order.
+hipError_t launchNaivePersistent(const DevScene& s, const PassParams& p, float4* frame, float4* res, int resPasses,
+                                 int* pixelCounter, int threshold, int blocks, unsigned long long* counters,
+                                 hipStream_t st);
 int naivePersistentBlocksPerCU();
 int naiveBatchBlocksPerCU();
 hipError_t launchNaiveBatch(const BatchScene* items, int nScenes, int* sceneCounters, int threshold, int blocks,

@@ -16,6 +16,8 @@
 // of random-number use.
 #include "zgl_internal.h"
 
+#include <algorithm>
+
 namespace zgl {
 
 namespace {
@@ -59,12 +61,17 @@ template <bool C> __device__ void flushCountersP(const Counters& cn, unsigned lo
     atomicAdd(out + 3, (unsigned long long)cn.traversals);
 }
 
-// The persistent schedule.  BATCH = false: one scene (s0, prm0, frame0, counter0).
+// The persistent schedule.
+// BATCH = false: one scene (s0, prm0, counter0).  The work is one sample per (pixel,
+// pass), handed out in (8x8 tile, pass) groups; each sample's value goes to
+// res[pass * W * H + pixel] and k_accumulate adds the passes to the frame in order, so
+// that no lane is left with a long run of one pixel's passes when the work runs out.
 // BATCH = true: the scenes of `items` in order; a wave works on one scene at a time
-// (scene data stays wave-uniform), takes its pixels from that scene's counter, and
-// moves to the next scene once its lanes have finished the pixels they hold.
+// (scene data stays wave-uniform), takes whole pixels (all passes, summed in a register)
+// from that scene's counter, and moves to the next scene once its lanes have finished
+// the pixels they hold.
 template <bool C, bool BATCH>
-__device__ __forceinline__ void naiveBody(const DevScene& s0, const PassParams& prm0, float4* frame0, int* counter0,
+__device__ __forceinline__ void naiveBody(const DevScene& s0, const PassParams& prm0, float4* res0, int* counter0,
                                           const BatchScene* items, int nScenes, int* sceneCounters, int threshold,
                                           unsigned long long* counters)
 {
@@ -98,10 +105,12 @@ __device__ __forceinline__ void naiveBody(const DevScene& s0, const PassParams& 
         const int csu = BATCH ? __builtin_amdgcn_readfirstlane(cs) : 0;
         const DevScene& s = BATCH ? items[csu].s : s0;
         const PassParams& prm = BATCH ? items[csu].prm : prm0;
-        float4* const frame = BATCH ? items[csu].frame : frame0;
+        float4* const frame = BATCH ? items[csu].frame : nullptr;
         int* const pixelCounter = BATCH ? sceneCounters + csu : counter0;
         const int W = s.filmW, H = s.filmH;
         const int nIndex = ((W + 7) / 8) * ((H + 7) / 8) * 64;
+        // non-batch work units: (tile, pass, pixel of the tile)
+        const int nUnits = BATCH ? nIndex : nIndex * prm.nPasses;
         const bool uniform = prm.lightEnvUniform != 0;
         const float portion = prm.lightPortion;
 
@@ -271,7 +280,11 @@ __device__ __forceinline__ void naiveBody(const DevScene& s0, const PassParams& 
                 }
             }
 
-            if (phase == kFinish) {
+            if (!BATCH && phase == kFinish) {
+                res0[(size_t)pass * ((size_t)W * H) + pix] = make_float4(result.x, result.y, result.z, 0.0f);
+                phase = kNewPixel;
+            }
+            if (BATCH && phase == kFinish) {
                 if (!hasNan(result)) {
                     const int spp = prm.firstPass + pass;
                     const vec3 last = (spp == 0) ? v3(0.0f) : v3(mem.x, mem.y, mem.z);
@@ -290,7 +303,7 @@ __device__ __forceinline__ void naiveBody(const DevScene& s0, const PassParams& 
         // is used up moves on to the next, and stops once all eight are.
         const unsigned long long need = __ballot(!walking && phase == kNewPixel);
         if (need) {
-            if (!walking && phase == kNewPixel && pix >= 0 && stored) frame[pix] = mem;
+            if (BATCH && !walking && phase == kNewPixel && pix >= 0 && stored) frame[pix] = mem;
             const int n = __popcll(need);
             const int leader = __ffsll((long long)need) - 1;
             int base = 0;
@@ -300,9 +313,9 @@ __device__ __forceinline__ void naiveBody(const DevScene& s0, const PassParams& 
                 base = __shfl(base, leader);
             } else {
                 // the wave's region of the image first (one per XCD), then the others
-                const int len = ((nIndex / kRegions + 63) / 64) * 64;
-                const int start = min(nIndex, region * len);
-                limit = min(nIndex, start + len);
+                const int len = ((nUnits / kRegions + 63) / 64) * 64;
+                const int start = min(nUnits, region * len);
+                limit = min(nUnits, start + len);
                 if (lane == leader) base = atomicAdd(pixelCounter + region, n);
                 base = __shfl(base, leader) + start;
                 if (base >= limit) {
@@ -322,14 +335,18 @@ __device__ __forceinline__ void naiveBody(const DevScene& s0, const PassParams& 
                 } else if (!BATCH && idx >= limit) {
                     // this region is used up: fetch again from the next one
                 } else {
+                    // batch: idx is a pixel of the tile order; otherwise a (tile, pass) unit
+                    const int group = idx >> 6;
+                    const int unitPass = BATCH ? 0 : group % prm.nPasses;
+                    const int pixIdx = BATCH ? idx : (group / prm.nPasses) * 64 + (idx & 63);
                     int x, y;
-                    pixelOfIndex(idx, W, x, y);
+                    pixelOfIndex(pixIdx, W, x, y);
                     if (x < W && y < H) {  // slots of partial edge tiles fetch again next round
                         pix = y * W + x;
                         scr = v2((float)x, (float)y) / v2((float)W, (float)H);
                         ps = pixelSeed(s, scr);
-                        mem = frame[pix];
-                        pass = 0;
+                        if (BATCH) mem = frame[pix];
+                        pass = unitPass;
                         phase = kNewSample;
                     }
                 }
@@ -403,11 +420,32 @@ __device__ __forceinline__ void naiveBody(const DevScene& s0, const PassParams& 
 }
 
 template <bool C>
-__global__ __launch_bounds__(256, 4) void k_naive_persistent(DevScene s, PassParams prm, float4* __restrict__ frame,
+__global__ __launch_bounds__(256, 4) void k_naive_persistent(DevScene s, PassParams prm, float4* __restrict__ res,
                                                           int* __restrict__ pixelCounter, int threshold,
                                                           unsigned long long* counters)
 {
-    naiveBody<C, false>(s, prm, frame, pixelCounter, nullptr, 0, nullptr, threshold, counters);
+    naiveBody<C, false>(s, prm, res, pixelCounter, nullptr, 0, nullptr, threshold, counters);
+}
+
+// A launch's samples into the frame, pass by pass in order: a NaN sample is skipped and
+// pass 0 replaces the frame (path_integ_naive.glsl:168-173).
+__global__ __launch_bounds__(256) void k_accumulate(const float4* __restrict__ res, int n, int nPasses, int firstPass,
+                                                    float4* __restrict__ frame)
+{
+    const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= n) return;
+    float4 m = frame[pix];
+    bool stored = false;
+    for (int k = 0; k < nPasses; k++) {
+        const float4 r = res[(size_t)k * n + pix];
+        const vec3 result = v3(r.x, r.y, r.z);
+        if (hasNan(result)) continue;
+        const vec3 last = (firstPass + k == 0) ? v3(0.0f) : v3(m.x, m.y, m.z);
+        const vec3 sum = last + result;
+        m = make_float4(sum.x, sum.y, sum.z, 1.0f);
+        stored = true;
+    }
+    if (stored) frame[pix] = m;
 }
 
 __global__ __launch_bounds__(256, 4) void k_naive_batch(const BatchScene* __restrict__ items, int nScenes,
@@ -428,18 +466,29 @@ __global__ void k_reset_counters(int* c, int n)
 
 namespace zgl {
 
-hipError_t launchNaivePersistent(const DevScene& s, const PassParams& p, float4* frame, int* pixelCounter,
-                                 int threshold, int blocks, unsigned long long* counters, hipStream_t st)
+hipError_t launchNaivePersistent(const DevScene& s, const PassParams& p, float4* frame, float4* res, int resPasses,
+                                 int* pixelCounter, int threshold, int blocks, unsigned long long* counters,
+                                 hipStream_t st)
 {
     if (threshold < 0) threshold = 0;
     if (threshold > 63) threshold = 63;
-    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, pixelCounter);
-    if (counters)
-        hipLaunchKernelGGL(k_naive_persistent<true>, dim3(blocks), dim3(256), 0, st, s, p, frame, pixelCounter,
-                           threshold, counters);
-    else
-        hipLaunchKernelGGL(k_naive_persistent<false>, dim3(blocks), dim3(256), 0, st, s, p, frame, pixelCounter,
-                           threshold, counters);
+    const int n = s.filmW * s.filmH;
+    // passes in chunks of at most resPasses (the capacity of res)
+    for (int k0 = 0; k0 < p.nPasses; k0 += resPasses) {
+        PassParams q = p;
+        q.firstPass = p.firstPass + k0;
+        q.freeCounter = p.freeCounter + k0;
+        q.nPasses = std::min(resPasses, p.nPasses - k0);
+        hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, pixelCounter);
+        if (counters)
+            hipLaunchKernelGGL(k_naive_persistent<true>, dim3(blocks), dim3(256), 0, st, s, q, res, pixelCounter,
+                               threshold, counters);
+        else
+            hipLaunchKernelGGL(k_naive_persistent<false>, dim3(blocks), dim3(256), 0, st, s, q, res, pixelCounter,
+                               threshold, counters);
+        hipLaunchKernelGGL(k_accumulate, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, res, n, q.nPasses,
+                           q.firstPass, frame);
+    }
     return hipGetLastError();
 }
 
