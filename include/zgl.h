@@ -158,7 +158,7 @@ typedef struct zgl_pass_params {
     int32_t loops_per_pass;       /* triple: LPTLoopsPerPass */
     int32_t count_bytes;          /* accumulate algorithmic-byte counters (slower) */
     int32_t regen_threshold;      /* naive: lanes of a wave64 still walking the MTBVH when the
-                                     wave stops to shade and reissue rays (1..63; 0 = default 10) */
+                                     wave stops to shade and reissue rays (1..63; 0 = default 20) */
 } zgl_pass_params;
 /* ZGL_INTEGRATOR_GLOBAL_QUEUE reads first_pass, n_passes, free_counter, max_depth,
  * russian_roulette, sample_light, light_env_uniform and light_portion; every contribution
@@ -228,7 +228,7 @@ void zgl_batch_destroy(zgl_batch* batch);
 const char* zgl_batch_last_error(const zgl_batch* batch);
 /* The scenes (contexts) and their passes (naive integrator); order = as given. */
 zgl_status zgl_batch_set(zgl_batch* batch, zgl_context* const* contexts, const zgl_pass_params* params, int32_t n);
-/* regen_threshold 1..63 (0 = default 10); blocks_per_cu 1..max (0 = max resident);
+/* regen_threshold 1..63 (0 = default 20); blocks_per_cu 1..max (0 = max resident);
  * order: a permutation of 0..n-1 giving the launch order (NULL keeps the current one). */
 zgl_status zgl_batch_configure(zgl_batch* batch, int32_t regen_threshold, int32_t blocks_per_cu,
                                const int32_t* order);

@@ -15,7 +15,9 @@
 using namespace zgl;
 
 // Lanes still walking when a wave of the persistent schedule stops to shade.
-constexpr int kDefaultRegenThreshold = 10;
+constexpr int kDefaultRegenThreshold = 20;
+// Largest total of a batch's per-sample value buffers.
+constexpr size_t kBatchSampleBytes = (size_t)64 << 30;
 
 namespace {
 
@@ -113,7 +115,9 @@ struct zgl_batch {
     std::vector<zgl_pass_params> params;
     std::vector<BatchScene> host;  // in launch order
     std::vector<int32_t> order;    // launch position -> index in ctxs
-    DevBuf items, counters, sobolTab;
+    DevBuf items, counters, sobolTab, res;
+    std::vector<size_t> resOffset;  // per context (index in ctxs): float4 offset into res
+    int maxPixels = 0;
     int numCUs = 0, maxBlocksPerCU = 1;
     int threshold = kDefaultRegenThreshold, blocksPerCU = 0;
     int sobolFirst = 0, sobolCount = 0;
@@ -841,6 +845,7 @@ void zgl_batch_destroy(zgl_batch* b)
     b->items.release();
     b->counters.release();
     b->sobolTab.release();
+    b->res.release();
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
@@ -874,6 +879,7 @@ static void batchUpload(zgl_batch* b)
         it.prm.pathsPerPass = 1536;
         it.prm.lptScale = 0.0f;
         it.frame = (float4*)ctx->frame.p;
+        it.res = (float4*)b->res.p + b->resOffset[i];
     }
     b->dropGraph();
     hipCheck(hipStreamSynchronize(b->stream), "batch");
@@ -914,6 +920,19 @@ zgl_status zgl_batch_set(zgl_batch* b, zgl_context* const* ctxs, const zgl_pass_
                                   b->sobolCount, b->stream),
                  "sobol table");
         b->ensure(b->counters, (size_t)n * 4, "batch counters");
+        // every scene's sample values (n_passes x W x H float4), added to its frame in
+        // pass order after the persistent kernel
+        b->resOffset.resize(n);
+        size_t total = 0;
+        b->maxPixels = 0;
+        for (int i = 0; i < n; i++) {
+            const int pixels = ctxs[i]->filmW * ctxs[i]->filmH;
+            b->resOffset[i] = total;
+            total += (size_t)params[i].n_passes * pixels;
+            b->maxPixels = std::max(b->maxPixels, pixels);
+        }
+        if (total * 16 > kBatchSampleBytes) invalid("batch: sample values exceed 64 GiB; split the batch");
+        b->ensure(b->res, total * 16, "batch sample values");
         batchUpload(b);
     });
 }
@@ -945,9 +964,9 @@ zgl_status zgl_batch_launch(zgl_batch* b)
         const int n = (int)b->ctxs.size();
         if (n == 0) throw ZglError{ZGL_E_STATE, "batch: no scenes set"};
         if (!b->exec) {
-            // the whole launch (counter reset + persistent kernel) as one graph
+            // the whole launch (counter reset, persistent kernel, accumulation) as one graph
             hipCheck(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "capture");
-            const hipError_t e = launchNaiveBatch((const BatchScene*)b->items.p, n, (int*)b->counters.p,
+            const hipError_t e = launchNaiveBatch((const BatchScene*)b->items.p, n, b->maxPixels, (int*)b->counters.p,
                                                   b->threshold, b->numCUs * b->blocksPerCU, b->stream);
             hipGraph_t g = nullptr;
             const hipError_t e2 = hipStreamEndCapture(b->stream, &g);

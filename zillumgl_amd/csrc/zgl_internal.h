@@ -34,6 +34,7 @@ struct BatchScene {
     DevScene s;
     PassParams prm;
     float4* frame;
+    float4* res;  // prm.nPasses x filmW x filmH sample values
 };
 
 hipError_t launchSobolTable(const uint32_t* sobolM, uint32_t* tab, int first, int count, hipStream_t st);
@@ -52,8 +53,9 @@ hipError_t launchNaivePersistent(const DevScene& s, const PassParams& p, float4*
                                  hipStream_t st);
 int naivePersistentBlocksPerCU();
 int naiveBatchBlocksPerCU();
-hipError_t launchNaiveBatch(const BatchScene* items, int nScenes, int* sceneCounters, int threshold, int blocks,
-                            hipStream_t st);
+// maxPixels: the largest filmW x filmH of the batch (sizes the accumulate launch).
+hipError_t launchNaiveBatch(const BatchScene* items, int nScenes, int maxPixels, int* sceneCounters, int threshold,
+                            int blocks, hipStream_t st);
 // Differentiable naive estimator: dFrame == null renders the replay estimator into
 // frame; otherwise accumulates FP64 gradients into dMat (nMaterials x 5) and the envCopies
 // copies of dEnv, summed into envSum (envW x envH x 3).

@@ -61,15 +61,14 @@ template <bool C> __device__ void flushCountersP(const Counters& cn, unsigned lo
     atomicAdd(out + 3, (unsigned long long)cn.traversals);
 }
 
-// The persistent schedule.
-// BATCH = false: one scene (s0, prm0, counter0).  The work is one sample per (pixel,
-// pass), handed out in (8x8 tile, pass) groups; each sample's value goes to
-// res[pass * W * H + pixel] and k_accumulate adds the passes to the frame in order, so
-// that no lane is left with a long run of one pixel's passes when the work runs out.
+// The persistent schedule.  The work is one sample per (pixel, pass), handed out in
+// (8x8 tile, pass) groups; each sample's value goes to res[pass * W * H + pixel] and
+// k_accumulate adds the passes to the frame in order, so that no lane is left with a
+// long run of one pixel's passes when the work runs out.
+// BATCH = false: one scene (s0, prm0, res0, counter0).
 // BATCH = true: the scenes of `items` in order; a wave works on one scene at a time
-// (scene data stays wave-uniform), takes whole pixels (all passes, summed in a register)
-// from that scene's counter, and moves to the next scene once its lanes have finished
-// the pixels they hold.
+// (scene data stays wave-uniform), takes its samples from that scene's counter, and
+// moves to the next scene once its lanes have finished the samples they hold.
 template <bool C, bool BATCH>
 __device__ __forceinline__ void naiveBody(const DevScene& s0, const PassParams& prm0, float4* res0, int* counter0,
                                           const BatchScene* items, int nScenes, int* sceneCounters, int threshold,
@@ -83,8 +82,6 @@ __device__ __forceinline__ void naiveBody(const DevScene& s0, const PassParams& 
 
     int phase = kNewPixel;
     int pix = -1, pass = 0;
-    float4 mem = make_float4(0.f, 0.f, 0.f, 0.f);
-    bool stored = false;
     vec2 scr = v2(0.f, 0.f);
     PixelSeed ps = {0u, 0u};
     Rng rng = {0u, 0u, 0, 0};
@@ -105,12 +102,12 @@ __device__ __forceinline__ void naiveBody(const DevScene& s0, const PassParams& 
         const int csu = BATCH ? __builtin_amdgcn_readfirstlane(cs) : 0;
         const DevScene& s = BATCH ? items[csu].s : s0;
         const PassParams& prm = BATCH ? items[csu].prm : prm0;
-        float4* const frame = BATCH ? items[csu].frame : nullptr;
+        float4* const res = BATCH ? items[csu].res : res0;
         int* const pixelCounter = BATCH ? sceneCounters + csu : counter0;
         const int W = s.filmW, H = s.filmH;
         const int nIndex = ((W + 7) / 8) * ((H + 7) / 8) * 64;
-        // non-batch work units: (tile, pass, pixel of the tile)
-        const int nUnits = BATCH ? nIndex : nIndex * prm.nPasses;
+        // work units: (tile, pass, pixel of the tile)
+        const int nUnits = nIndex * prm.nPasses;
         const bool uniform = prm.lightEnvUniform != 0;
         const float portion = prm.lightPortion;
 
@@ -280,20 +277,9 @@ __device__ __forceinline__ void naiveBody(const DevScene& s0, const PassParams& 
                 }
             }
 
-            if (!BATCH && phase == kFinish) {
-                res0[(size_t)pass * ((size_t)W * H) + pix] = make_float4(result.x, result.y, result.z, 0.0f);
+            if (phase == kFinish) {
+                res[(size_t)pass * ((size_t)W * H) + pix] = make_float4(result.x, result.y, result.z, 0.0f);
                 phase = kNewPixel;
-            }
-            if (BATCH && phase == kFinish) {
-                if (!hasNan(result)) {
-                    const int spp = prm.firstPass + pass;
-                    const vec3 last = (spp == 0) ? v3(0.0f) : v3(mem.x, mem.y, mem.z);
-                    const vec3 sum = last + result;
-                    mem = make_float4(sum.x, sum.y, sum.z, 1.0f);
-                    stored = true;
-                }
-                pass++;
-                phase = (pass < prm.nPasses) ? kNewSample : kNewPixel;
             }
         }
 
@@ -303,11 +289,10 @@ __device__ __forceinline__ void naiveBody(const DevScene& s0, const PassParams& 
         // is used up moves on to the next, and stops once all eight are.
         const unsigned long long need = __ballot(!walking && phase == kNewPixel);
         if (need) {
-            if (BATCH && !walking && phase == kNewPixel && pix >= 0 && stored) frame[pix] = mem;
             const int n = __popcll(need);
             const int leader = __ffsll((long long)need) - 1;
             int base = 0;
-            int limit = nIndex;
+            int limit = nUnits;
             if (BATCH) {
                 if (lane == leader) base = atomicAdd(pixelCounter, n);
                 base = __shfl(base, leader);
@@ -329,23 +314,20 @@ __device__ __forceinline__ void naiveBody(const DevScene& s0, const PassParams& 
                 const unsigned long long below = need & ((1ull << lane) - 1ull);
                 const int idx = base + __popcll(below);
                 pix = -1;
-                stored = false;
-                if (BATCH ? idx >= nIndex : exhausted >= kRegions) {
+                if (BATCH ? idx >= nUnits : exhausted >= kRegions) {
                     phase = BATCH ? kWait : kDone;
                 } else if (!BATCH && idx >= limit) {
                     // this region is used up: fetch again from the next one
                 } else {
-                    // batch: idx is a pixel of the tile order; otherwise a (tile, pass) unit
                     const int group = idx >> 6;
-                    const int unitPass = BATCH ? 0 : group % prm.nPasses;
-                    const int pixIdx = BATCH ? idx : (group / prm.nPasses) * 64 + (idx & 63);
+                    const int unitPass = group % prm.nPasses;
+                    const int pixIdx = (group / prm.nPasses) * 64 + (idx & 63);
                     int x, y;
                     pixelOfIndex(pixIdx, W, x, y);
                     if (x < W && y < H) {  // slots of partial edge tiles fetch again next round
                         pix = y * W + x;
                         scr = v2((float)x, (float)y) / v2((float)W, (float)H);
                         ps = pixelSeed(s, scr);
-                        if (BATCH) mem = frame[pix];
                         pass = unitPass;
                         phase = kNewSample;
                     }
@@ -429,11 +411,9 @@ __global__ __launch_bounds__(256, 4) void k_naive_persistent(DevScene s, PassPar
 
 // A launch's samples into the frame, pass by pass in order: a NaN sample is skipped and
 // pass 0 replaces the frame (path_integ_naive.glsl:168-173).
-__global__ __launch_bounds__(256) void k_accumulate(const float4* __restrict__ res, int n, int nPasses, int firstPass,
-                                                    float4* __restrict__ frame)
+__device__ __forceinline__ void accumulatePixel(const float4* __restrict__ res, int pix, int n, int nPasses,
+                                                int firstPass, float4* __restrict__ frame)
 {
-    const int pix = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pix >= n) return;
     float4 m = frame[pix];
     bool stored = false;
     for (int k = 0; k < nPasses; k++) {
@@ -446,6 +426,24 @@ __global__ __launch_bounds__(256) void k_accumulate(const float4* __restrict__ r
         stored = true;
     }
     if (stored) frame[pix] = m;
+}
+
+__global__ __launch_bounds__(256) void k_accumulate(const float4* __restrict__ res, int n, int nPasses, int firstPass,
+                                                    float4* __restrict__ frame)
+{
+    const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix < n) accumulatePixel(res, pix, n, nPasses, firstPass, frame);
+}
+
+// The same for every scene of a batch: blockIdx.y (strided by gridDim.y) picks the scene.
+__global__ __launch_bounds__(256) void k_accumulate_batch(const BatchScene* __restrict__ items, int nScenes)
+{
+    const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int k = blockIdx.y; k < nScenes; k += gridDim.y) {
+        const BatchScene& it = items[k];
+        const int n = it.s.filmW * it.s.filmH;
+        if (pix < n) accumulatePixel(it.res, pix, n, it.prm.nPasses, it.prm.firstPass, it.frame);
+    }
 }
 
 __global__ __launch_bounds__(256, 4) void k_naive_batch(const BatchScene* __restrict__ items, int nScenes,
@@ -492,13 +490,15 @@ hipError_t launchNaivePersistent(const DevScene& s, const PassParams& p, float4*
     return hipGetLastError();
 }
 
-hipError_t launchNaiveBatch(const BatchScene* items, int nScenes, int* sceneCounters, int threshold, int blocks,
-                            hipStream_t st)
+hipError_t launchNaiveBatch(const BatchScene* items, int nScenes, int maxPixels, int* sceneCounters, int threshold,
+                            int blocks, hipStream_t st)
 {
     if (threshold < 0) threshold = 0;
     if (threshold > 63) threshold = 63;
     hipLaunchKernelGGL(k_reset_counters, dim3((nScenes + 255) / 256), dim3(256), 0, st, sceneCounters, nScenes);
     hipLaunchKernelGGL(k_naive_batch, dim3(blocks), dim3(256), 0, st, items, nScenes, sceneCounters, threshold);
+    hipLaunchKernelGGL(k_accumulate_batch, dim3((unsigned)((maxPixels + 255) / 256), (unsigned)std::min(nScenes, 4096)),
+                       dim3(256), 0, st, items, nScenes);
     return hipGetLastError();
 }
 
