@@ -160,7 +160,10 @@ typedef struct zgl_pass_params {
     int32_t regen_threshold;      /* naive: lanes of a wave64 still walking the MTBVH when the
                                      wave stops to shade and reissue rays (1..63; 0 = default 20) */
 } zgl_pass_params;
-/* ZGL_INTEGRATOR_GLOBAL_QUEUE reads first_pass, n_passes, free_counter, max_depth,
+/* ZGL_INTEGRATOR_NAIVE keeps each sample's value (16 bytes) in a context-owned buffer of
+ * at most 4 GiB and adds them to the frame in pass order; calls with more passes than fit
+ * run in chunks of passes.
+ * ZGL_INTEGRATOR_GLOBAL_QUEUE reads first_pass, n_passes, free_counter, max_depth,
  * russian_roulette, sample_light, light_env_uniform and light_portion; every contribution
  * is added to the frame as produced (accumulateImage), so the frame must be cleared
  * (zgl_clear_frame) on reset, as the reference's image clear does.  count_bytes is not
@@ -226,7 +229,9 @@ typedef struct zgl_batch zgl_batch;
 zgl_status zgl_batch_create(int32_t device, zgl_batch** out);
 void zgl_batch_destroy(zgl_batch* batch);
 const char* zgl_batch_last_error(const zgl_batch* batch);
-/* The scenes (contexts) and their passes (naive integrator); order = as given. */
+/* The scenes (contexts) and their passes (naive integrator); order = as given.  Allocates
+ * the batch's sample values, sum of n_passes x width x height x 16 bytes; a batch above
+ * 64 GiB of them is rejected (ZGL_E_INVALID). */
 zgl_status zgl_batch_set(zgl_batch* batch, zgl_context* const* contexts, const zgl_pass_params* params, int32_t n);
 /* regen_threshold 1..63 (0 = default 20); blocks_per_cu 1..max (0 = max resident);
  * order: a permutation of 0..n-1 giving the launch order (NULL keeps the current one). */
