@@ -213,7 +213,10 @@ zgl_status zgl_render_replay(zgl_context* ctx, const zgl_pass_params* params);
 /* Path-replay adjoint: gradients of sum(d_frame . frame sum of these passes) with
  * respect to the parameters at the evaluation point.  d_frame: W*H*4 floats (RGBA
  * layout of zgl_read_frame, alpha ignored).  Outputs are overwritten: d_material_params
- * n_materials x 5, d_env_rgb env_height x env_width x 3.  Does not change the frame. */
+ * n_materials x 5, d_env_rgb env_height x env_width x 3.  Does not change the frame.
+ * Works in chunks of passes: the forward tracer logs each sample's traversal results
+ * (up to 16 queries, 128 bytes per sample, with the sample values at most 8 GiB per
+ * chunk) and the replay differentiates from the log. */
 zgl_status zgl_adjoint(zgl_context* ctx, const zgl_pass_params* params, const float* d_frame,
                        float* d_material_params, float* d_env_rgb);
 

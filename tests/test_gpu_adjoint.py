@@ -116,6 +116,31 @@ def test_adjoint_matches_oracle_fp64(ctx, c2s):
     assert em <= 5e-3 and ee <= 1e-3
 
 
+def test_adjoint_offset_passes_and_split_launches(ctx, c2s):
+    """Passes that start past 0 (first_pass 5, free_counter 9) against the oracle, with
+    Russian roulette on; and the gradients of passes [5, 11) equal those of [5, 8) plus
+    [8, 11) (each launch logs and differentiates its own passes), up to the order of the
+    FP64 atomic sums."""
+    print("[gpu] adjoint: offset passes vs oracle, split launches", flush=True)
+    ctx.upload(c2s)
+    rng = np.random.default_rng(6)
+    _, _, p1, e1 = _perturb(c2s, 6)
+    d_frame = rng.uniform(-1, 1, (c2s.height, c2s.width, 3)).astype(np.float32)
+    ctx.set_eval_params(p1, e1)
+    gm, ge = ctx.adjoint(d_frame, p1.shape[0], e1.shape, 5, 6, 9, DEPTH, russian_roulette=True)
+    rm, re = orc.adjoint(c2s, orc.params(0, 5, 6, 9, DEPTH, True), _records(c2s, p1), e1, d_frame, fp32_values=True)
+    em, nm = _close(gm, rm, 1e-3)
+    ee, ne = _close(ge, re, 1e-3)
+    print(f"[gpu] adjoint offset: material max rel err {em:.3g} over {nm}, env {ee:.3g} over {ne}", flush=True)
+    assert nm >= 20 and ne >= 1000
+    assert em <= 5e-3 and ee <= 1e-3
+    am, ae = ctx.adjoint(d_frame, p1.shape[0], e1.shape, 5, 3, 9, DEPTH, russian_roulette=True)
+    bm, be = ctx.adjoint(d_frame, p1.shape[0], e1.shape, 8, 3, 12, DEPTH, russian_roulette=True)
+    for whole, parts in ((gm, am.astype(np.float64) + bm), (ge, ae.astype(np.float64) + be)):
+        scale = np.abs(whole).max()
+        assert np.abs(whole - parts).max() <= 1e-5 * scale
+
+
 def test_adjoint_deep_paths_match_oracle_fp64(ctx, c2s):
     """Depth 10 issues up to 21 traversal queries per sample, more than the replay's hit
     log holds (16): the queries past it are walked again in the replay pass."""

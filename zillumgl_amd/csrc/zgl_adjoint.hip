@@ -135,20 +135,27 @@ ZGL_D void acc(Sum3& a, vec3 t, double sign)
     a.z += sign * (double)t.z;
 }
 
-// Traversal results of one sample.  The first pass records every closest-hit query
-// (primitive, distance) and every shadow query (occluded) in order; the replay pass,
-// which issues the same queries with the same rays, reads them back instead of walking
-// the MTBVH again.  Queries past the log's capacity (paths deeper than the log holds)
-// are walked again in the replay, which gives the same results.
-constexpr int kHitLog = 16;
+// Traversal results of one sample.  With `pre` (the forward kernel's log of the sample,
+// written by launchNaiveLogged), both passes read the results from it.  Without, the
+// first pass records every closest-hit query (primitive, distance) and every shadow
+// query (occluded) in order, and the replay pass reads them back.  Either way the queries
+// are the same, with the same rays, as the walks that produced the log, so are the
+// results.  Queries past the log's capacity (paths deeper than the log holds) are walked.
+constexpr int kHitLog = kHitLogQueries;
 struct HitLog {
     int n;
+    const int2* pre;
     int id[kHitLog];
     float dist[kHitLog];
 };
 template <bool ADJ> ZGL_D int loggedHit(const DevScene& s, const Ray& r, float& dist, HitLog& log)
 {
     const int i = log.n++;
+    if (log.pre && i < kHitLog) {
+        const int2 e = log.pre[i];
+        dist = __int_as_float(e.y);
+        return e.x;
+    }
     if (ADJ && i < kHitLog) {
         dist = log.dist[i];
         return log.id[i];
@@ -164,6 +171,7 @@ template <bool ADJ> ZGL_D int loggedHit(const DevScene& s, const Ray& r, float& 
 template <bool ADJ> ZGL_D bool loggedTest(const DevScene& s, const Ray& r, float tmax, HitLog& log)
 {
     const int i = log.n++;
+    if (log.pre && i < kHitLog) return log.pre[i].x != 0;
     if (ADJ && i < kHitLog) return log.id[i] != 0;
     Counters cn;
     const bool occluded = bvhTest<false>(s, r, tmax, cn);
@@ -335,7 +343,8 @@ ZGL_D vec3 replayPath(const DevScene& s, const Replay& rp, const PassParams& prm
 template <int MODE>
 __global__ __launch_bounds__(256) void k_naive_replay(DevScene s, PassParams prm, Replay rp, float4* __restrict__ frame,
                                                       const float4* __restrict__ dFrame, double* __restrict__ dMat,
-                                                      double* __restrict__ dEnv, int envCopies, int passChunk)
+                                                      double* __restrict__ dEnv, int envCopies, int passChunk,
+                                                      const int2* __restrict__ hitLog)
 {
     const bool useLds = MODE == 1 && s.nMaterials <= kLdsMaterials;
     if (useLds) {
@@ -362,7 +371,9 @@ __global__ __launch_bounds__(256) void k_naive_replay(DevScene s, PassParams prm
         const int k0 = MODE == 1 ? (int)blockIdx.z * passChunk : 0;
         const int k1 = MODE == 1 ? min(prm.nPasses, k0 + passChunk) : prm.nPasses;
         HitLog log;
+        log.pre = nullptr;
         for (int k = k0; k < k1; k++) {
+            if (hitLog) log.pre = hitLog + ((size_t)k * ((size_t)s.filmW * s.filmH) + pix) * kHitLog;
             const int spp = prm.firstPass + k;
             const uint32_t fc = (uint32_t)(prm.freeCounter + k);
             Rng rng;
@@ -426,13 +437,13 @@ constexpr int kPassChunk = 8;
 
 hipError_t launchNaiveReplay(const DevScene& s, const PassParams& p, const float4* evalMat, const float4* evalEnv,
                              float4* frame, const float4* dFrame, double* dMat, double* dEnv, int envCopies,
-                             double* envSum, hipStream_t st)
+                             double* envSum, const int2* hitLog, hipStream_t st)
 {
     const Replay rp{evalMat, evalEnv};
     const dim3 grid((s.filmW + 15) / 16, (s.filmH + 15) / 16);
     if (!dFrame) {
         hipLaunchKernelGGL(k_naive_replay<0>, grid, dim3(256), 0, st, s, p, rp, frame, dFrame, dMat, dEnv, 1,
-                           p.nPasses);
+                           p.nPasses, hitLog);
         return hipGetLastError();
     }
     if (envCopies < 1) envCopies = 1;
@@ -440,7 +451,7 @@ hipError_t launchNaiveReplay(const DevScene& s, const PassParams& p, const float
     const int chunks = (p.nPasses + kPassChunk - 1) / kPassChunk;
     const dim3 gridAdj(grid.x, grid.y, (unsigned)chunks);
     hipLaunchKernelGGL(k_naive_replay<1>, gridAdj, dim3(256), 0, st, s, p, rp, frame, dFrame, dMat, dEnv, envCopies,
-                       kPassChunk);
+                       kPassChunk, hitLog);
     const int64_t n = (int64_t)s.envW * s.envH * 3;
     hipLaunchKernelGGL(k_sum_copies, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, dEnv, envSum, n,
                        envCopies);

@@ -16,6 +16,8 @@ using namespace zgl;
 
 // Lanes still walking when a wave of the persistent schedule stops to shade.
 constexpr int kDefaultRegenThreshold = 20;
+// Largest sample-value plus traversal-log buffer of one adjoint chunk of passes.
+constexpr size_t kAdjointLogBytes = (size_t)8 << 30;
 // Largest total of a batch's per-sample value buffers.
 constexpr size_t kBatchSampleBytes = (size_t)64 << 30;
 
@@ -84,6 +86,7 @@ struct zgl_context {
     int gqBlocksPerCU = 0;
     DevBuf wfWork;    // wavefront schedule of the naive tracer
     DevBuf naiveRes;  // per-sample values of the persistent naive tracer
+    DevBuf hitLog;    // per-sample traversal results for the adjoint
     int wfTraceBlocksPerCU = 0, wfShadeBlocksPerCU = 0;
     DeviceBvh bvh;     // work memory of the device MTBVH build
     std::vector<float> hostMaterials;  // uploaded material records (16 floats each)
@@ -101,7 +104,7 @@ struct zgl_context {
                          &lightPower, &lightAlias, &lightProb, &env, &envAlias, &envProb, &noise, &sobolM,
                          &sobolTab, &frame, &film, &counters, &scratchRays, &scratchOut, &workCounter,
                          &evalMat, &evalEnv, &dFrame, &dMat, &dEnv, &bvhBounds, &gqQueues, &gqCounts, &wfWork,
-                         &naiveRes};
+                         &naiveRes, &hitLog};
         for (DevBuf* b : all) b->release();
         bvh.release();
     }
@@ -685,7 +688,7 @@ zgl_status zgl_render_replay(zgl_context* ctx, const zgl_pass_params* p)
         preparePasses(ctx, p, s, pp);
         if (p->integrator != ZGL_INTEGRATOR_NAIVE) invalid("the replay estimator is the naive integrator's");
         hipCheck(launchNaiveReplay(s, pp, (const float4*)ctx->evalMat.p, (const float4*)ctx->evalEnv.p,
-                                   (float4*)ctx->frame.p, nullptr, nullptr, nullptr, 1, nullptr, ctx->stream),
+                                   (float4*)ctx->frame.p, nullptr, nullptr, nullptr, 1, nullptr, nullptr, ctx->stream),
                  "replay pass");
     });
 }
@@ -712,10 +715,28 @@ zgl_status zgl_adjoint(zgl_context* ctx, const zgl_pass_params* p, const float* 
         double* envSum = envCopies + (size_t)copies * ne;
         hipCheck(launchFill((float*)ctx->dMat.p, (int64_t)nm * 2, 0.0f, ctx->stream), "clear");
         hipCheck(launchFill((float*)envCopies, (int64_t)ne * 2 * copies, 0.0f, ctx->stream), "clear");
-        hipCheck(launchNaiveReplay(s, pp, (const float4*)ctx->evalMat.p, (const float4*)ctx->evalEnv.p,
-                                   (float4*)ctx->frame.p, (const float4*)ctx->dFrame.p, (double*)ctx->dMat.p,
-                                   envCopies, copies, envSum, ctx->stream),
-                 "adjoint pass");
+        // In chunks of passes: the persistent forward kernel logs every sample's traversal
+        // results (the rays depend only on the uploaded scene, which all sampling uses),
+        // then the replay kernel differentiates the chunk's samples from the log.
+        if (!ctx->naiveBlocksPerCU) ctx->naiveBlocksPerCU = naivePersistentBlocksPerCU();
+        const size_t perPass = px * (16 + (size_t)kHitLogQueries * 8);
+        const int chunk = (int)std::max<size_t>(1, std::min<size_t>(pp.nPasses, kAdjointLogBytes / perPass));
+        ctx->ensure(ctx->naiveRes, px * 16 * chunk, "sample values");
+        ctx->ensure(ctx->hitLog, px * kHitLogQueries * 8 * chunk, "traversal log");
+        for (int k0 = 0; k0 < pp.nPasses; k0 += chunk) {
+            PassParams q = pp;
+            q.firstPass = pp.firstPass + k0;
+            q.freeCounter = pp.freeCounter + k0;
+            q.nPasses = std::min(chunk, pp.nPasses - k0);
+            hipCheck(launchNaiveLogged(s, q, (float4*)ctx->naiveRes.p, (int2*)ctx->hitLog.p, (int*)ctx->workCounter.p,
+                                       p->regen_threshold > 0 ? p->regen_threshold : kDefaultRegenThreshold,
+                                       ctx->numCUs * ctx->naiveBlocksPerCU, ctx->stream),
+                     "logged forward pass");
+            hipCheck(launchNaiveReplay(s, q, (const float4*)ctx->evalMat.p, (const float4*)ctx->evalEnv.p,
+                                       (float4*)ctx->frame.p, (const float4*)ctx->dFrame.p, (double*)ctx->dMat.p,
+                                       envCopies, copies, envSum, (const int2*)ctx->hitLog.p, ctx->stream),
+                     "adjoint pass");
+        }
         std::vector<double> gm(nm), ge(ne);
         hipCheck(hipMemcpyAsync(gm.data(), ctx->dMat.p, nm * 8, hipMemcpyDeviceToHost, ctx->stream), "copy");
         hipCheck(hipMemcpyAsync(ge.data(), envSum, ne * 8, hipMemcpyDeviceToHost, ctx->stream), "copy");

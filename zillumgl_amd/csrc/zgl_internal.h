@@ -52,6 +52,13 @@ hipError_t launchNaivePersistent(const DevScene& s, const PassParams& p, float4*
                                  int* pixelCounter, int threshold, int blocks, unsigned long long* counters,
                                  hipStream_t st);
 int naivePersistentBlocksPerCU();
+// Traversal results logged per sample for the adjoint (see launchNaiveLogged).
+constexpr int kHitLogQueries = 16;
+// The persistent naive tracer over p's passes (all of them in one launch: res holds
+// p.nPasses x W x H values, hitLog p.nPasses x W x H x kHitLogQueries entries), writing
+// each sample's value to res and its traversal results to hitLog; the frame is untouched.
+hipError_t launchNaiveLogged(const DevScene& s, const PassParams& p, float4* res, int2* hitLog, int* pixelCounter,
+                             int threshold, int blocks, hipStream_t st);
 int naiveBatchBlocksPerCU();
 // maxPixels: the largest filmW x filmH of the batch (sizes the accumulate launch).
 hipError_t launchNaiveBatch(const BatchScene* items, int nScenes, int maxPixels, int* sceneCounters, int threshold,
@@ -59,9 +66,11 @@ hipError_t launchNaiveBatch(const BatchScene* items, int nScenes, int maxPixels,
 // Differentiable naive estimator: dFrame == null renders the replay estimator into
 // frame; otherwise accumulates FP64 gradients into dMat (nMaterials x 5) and the envCopies
 // copies of dEnv, summed into envSum (envW x envH x 3).
+// hitLog (adjoint only, may be null): the traversal results of p's passes written by
+// launchNaiveLogged; the replay then walks the MTBVH only for queries past the log.
 hipError_t launchNaiveReplay(const DevScene& s, const PassParams& p, const float4* evalMat, const float4* evalEnv,
                              float4* frame, const float4* dFrame, double* dMat, double* dEnv, int envCopies,
-                             double* envSum, hipStream_t st);
+                             double* envSum, const int2* hitLog, hipStream_t st);
 // Global-queue path tracer: device bytes of the two item queues for a w x h film, blocks
 // per CU of the extension kernel, and the launch of p.nPasses passes (counts: maxDepth + 1
 // ints of device memory).

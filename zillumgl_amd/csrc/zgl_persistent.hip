@@ -66,13 +66,18 @@ template <bool C> __device__ void flushCountersP(const Counters& cn, unsigned lo
 // k_accumulate adds the passes to the frame in order, so that no lane is left with a
 // long run of one pixel's passes when the work runs out.
 // BATCH = false: one scene (s0, prm0, res0, counter0).
+// LOG (single scene only): every traversal query's result is also written to
+// hitLog[(pass * W * H + pixel) * kHitLogQueries + query], in issue order (primary,
+// then per vertex the shadow query if one is cast and the extension), up to
+// kHitLogQueries per sample: {closest primitive, distance bits} for closest-hit queries,
+// {1 if occluded else 0, 0} for shadow queries.  The adjoint replays read it.
 // BATCH = true: the scenes of `items` in order; a wave works on one scene at a time
 // (scene data stays wave-uniform), takes its samples from that scene's counter, and
 // moves to the next scene once its lanes have finished the samples they hold.
-template <bool C, bool BATCH>
+template <bool C, bool BATCH, bool LOG>
 __device__ __forceinline__ void naiveBody(const DevScene& s0, const PassParams& prm0, float4* res0, int* counter0,
                                           const BatchScene* items, int nScenes, int* sceneCounters, int threshold,
-                                          unsigned long long* counters)
+                                          unsigned long long* counters, int2* hitLog)
 {
     const int lane = threadIdx.x & 63;
     Counters cn;
@@ -82,6 +87,7 @@ __device__ __forceinline__ void naiveBody(const DevScene& s0, const PassParams& 
 
     int phase = kNewPixel;
     int pix = -1, pass = 0;
+    int query = 0;  // LOG: traversal queries of the current sample so far
     vec2 scr = v2(0.f, 0.f);
     PixelSeed ps = {0u, 0u};
     Rng rng = {0u, 0u, 0, 0};
@@ -110,11 +116,23 @@ __device__ __forceinline__ void naiveBody(const DevScene& s0, const PassParams& 
         const int nUnits = nIndex * prm.nPasses;
         const bool uniform = prm.lightEnvUniform != 0;
         const float portion = prm.lightPortion;
+        // LOG: the result of the walk that just ended, as the next query of the sample
+        auto logQuery = [&](bool shadowQuery) {
+            if constexpr (LOG) {
+                if (query < kHitLogQueries) {
+                    const int2 e = shadowQuery ? make_int2(wk.closest >= 0 ? 1 : 0, 0)
+                                               : make_int2(wk.closest, __float_as_int(wk.dist));
+                    hitLog[((size_t)pass * ((size_t)W * H) + pix) * kHitLogQueries + query] = e;
+                }
+                query++;
+            }
+        };
 
         // ------------------------------------------------------------ shading
         if (!walking) {
             // completion of the ray that was walked
             if (phase == kPrimary) {
+                logQuery(false);
                 const int id = wk.closest;
                 const Ray ray = makeRay(wk.p.o, wk.p.d);
                 pos = rayPoint(ray, wk.dist);
@@ -132,6 +150,7 @@ __device__ __forceinline__ void naiveBody(const DevScene& s0, const PassParams& 
                     phase = (prm.maxDepth >= 1) ? kSurface : kFinish;
                 }
             } else if (phase == kShadow) {
+                logQuery(true);
                 if (wk.closest < 0) {
                     if (C) cn.bytes += pendBytes;
                     if (pendValid) result += pend;
@@ -143,6 +162,7 @@ __device__ __forceinline__ void naiveBody(const DevScene& s0, const PassParams& 
                     phase = kExtend;
                 }
             } else if (phase == kExtend) {
+                logQuery(false);
                 const int nextId = wk.closest;
                 const int lightId = nextId - s.objPrimCount;
                 const vec3 nextPos = rayPoint(makeRay(wk.p.o, wk.p.d), wk.dist);
@@ -342,6 +362,7 @@ __device__ __forceinline__ void naiveBody(const DevScene& s0, const PassParams& 
             rng.sampleSeed = ps.tx * ps.ty;
             rng.sampleOffset = sampleOffsetOf(spp);
             rng.dim = 0;
+            query = 0;
             if (C) cn.bytes += 64 + 32;
             const vec4 u = sample4D<C>(s, rng, cn);
             walkStart<C>(s, wk, cameraSampleRay(s, scr, u), false, 1e8f, cn);
@@ -381,6 +402,7 @@ __device__ __forceinline__ void naiveBody(const DevScene& s0, const PassParams& 
                 }
                 // a shadow walk that ended settles its NEE term and starts the extension walk
                 if (!walking && phase == kShadow) {
+                    logQuery(true);
                     if (wk.closest < 0) {
                         if (C) cn.bytes += pendBytes;
                         if (pendValid) result += pend;
@@ -401,12 +423,12 @@ __device__ __forceinline__ void naiveBody(const DevScene& s0, const PassParams& 
     flushCountersP<C>(cn, counters);
 }
 
-template <bool C>
+template <bool C, bool LOG>
 __global__ __launch_bounds__(256, 4) void k_naive_persistent(DevScene s, PassParams prm, float4* __restrict__ res,
                                                           int* __restrict__ pixelCounter, int threshold,
-                                                          unsigned long long* counters)
+                                                          unsigned long long* counters, int2* __restrict__ hitLog)
 {
-    naiveBody<C, false>(s, prm, res, pixelCounter, nullptr, 0, nullptr, threshold, counters);
+    naiveBody<C, false, LOG>(s, prm, res, pixelCounter, nullptr, 0, nullptr, threshold, counters, hitLog);
 }
 
 // A launch's samples into the frame, pass by pass in order: a NaN sample is skipped and
@@ -449,8 +471,8 @@ __global__ __launch_bounds__(256) void k_accumulate_batch(const BatchScene* __re
 __global__ __launch_bounds__(256, 4) void k_naive_batch(const BatchScene* __restrict__ items, int nScenes,
                                                      int* __restrict__ sceneCounters, int threshold)
 {
-    naiveBody<false, true>(items[0].s, items[0].prm, nullptr, nullptr, items, nScenes, sceneCounters, threshold,
-                           nullptr);
+    naiveBody<false, true, false>(items[0].s, items[0].prm, nullptr, nullptr, items, nScenes, sceneCounters,
+                                  threshold, nullptr, nullptr);
 }
 
 // Zeroes the per-scene pixel counters of a batch (a kernel, so that it can sit in a graph).
@@ -479,14 +501,25 @@ hipError_t launchNaivePersistent(const DevScene& s, const PassParams& p, float4*
         q.nPasses = std::min(resPasses, p.nPasses - k0);
         hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, pixelCounter);
         if (counters)
-            hipLaunchKernelGGL(k_naive_persistent<true>, dim3(blocks), dim3(256), 0, st, s, q, res, pixelCounter,
-                               threshold, counters);
+            hipLaunchKernelGGL((k_naive_persistent<true, false>), dim3(blocks), dim3(256), 0, st, s, q, res,
+                               pixelCounter, threshold, counters, nullptr);
         else
-            hipLaunchKernelGGL(k_naive_persistent<false>, dim3(blocks), dim3(256), 0, st, s, q, res, pixelCounter,
-                               threshold, counters);
+            hipLaunchKernelGGL((k_naive_persistent<false, false>), dim3(blocks), dim3(256), 0, st, s, q, res,
+                               pixelCounter, threshold, counters, nullptr);
         hipLaunchKernelGGL(k_accumulate, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, res, n, q.nPasses,
                            q.firstPass, frame);
     }
+    return hipGetLastError();
+}
+
+hipError_t launchNaiveLogged(const DevScene& s, const PassParams& p, float4* res, int2* hitLog, int* pixelCounter,
+                             int threshold, int blocks, hipStream_t st)
+{
+    if (threshold < 0) threshold = 0;
+    if (threshold > 63) threshold = 63;
+    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, pixelCounter);
+    hipLaunchKernelGGL((k_naive_persistent<false, true>), dim3(blocks), dim3(256), 0, st, s, p, res, pixelCounter,
+                       threshold, nullptr, hitLog);
     return hipGetLastError();
 }
 
@@ -512,7 +545,7 @@ int naiveBatchBlocksPerCU()
 int naivePersistentBlocksPerCU()
 {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_naive_persistent<false>, 256, 0) != hipSuccess) return 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_naive_persistent<false, false>, 256, 0) != hipSuccess) return 1;
     return n > 0 ? n : 1;
 }
 
