@@ -137,13 +137,15 @@ def adjoint_bench(hip, steps):
 
 def batch_bench(reps):
     """64 scenes (Cornell boxes tessellated to 1k-2M triangles, 128^2-1024^2, 4-64 spp,
-    depth 3-10) rendered three ways: (a) per scene and per pass, one zgl_render launch
-    each -- the reference's dispatch pattern (NaivePathIntegrator::renderOnePass,
-    src/integrator/NaivePath.cpp:77-104, one dispatch per sample pass); (b) per scene, one
-    zgl_render launch holding all its passes; (c) one batched persistent launch replayed
-    from a HIP graph, after the autotuner picked its scene order, regeneration threshold
-    and occupancy.  Frames are bit-identical between the three forms
-    (tests/test_gpu_batch.py, tests/test_gpu_parity.py), so RMSE is equal."""
+    depth 3-10) rendered four ways: (a) per scene, one launch of the plain kernel (one
+    thread per pixel running its passes, the shape of path_integ_naive.glsl; it is
+    zgl_render_replay at the uploaded parameters); (b) per scene and per pass, one
+    zgl_render launch each -- the reference's dispatch pattern
+    (NaivePathIntegrator::renderOnePass, src/integrator/NaivePath.cpp:77-104, one dispatch
+    per sample pass); (c) per scene, one zgl_render launch holding all its passes; (d) one
+    batched persistent launch replayed from a HIP graph, after the autotuner picked its
+    scene order, regeneration threshold and occupancy.  Frames are bit-identical between
+    the four forms (tests/test_gpu_batch.py), so RMSE is equal."""
     from concurrent.futures import ThreadPoolExecutor
     from zillumgl_amd import scene, render, _lib
     descs = scene.config4()
@@ -171,7 +173,12 @@ def batch_bench(reps):
                 c.render(_lib.INTEGRATOR_NAIVE, first + k, 1, fc + k, depth)
             c.synchronize()
 
-    def timed(fn):
+    def plain():
+        for c, (first, n, fc, depth) in zip(ctxs, passes):
+            c.render_replay(first, n, fc, depth)
+            c.synchronize()
+
+    def timed(fn, reps=reps):
         fn()
         ts = []
         for _ in range(reps):
@@ -179,7 +186,8 @@ def batch_bench(reps):
             fn()
             ts.append(time.perf_counter() - t)
         return sorted(ts)[len(ts) // 2]
-    t_pass = timed(per_pass)
+    t_plain = timed(plain, 1)
+    t_pass = timed(per_pass, 1)
     t_seq = timed(sequential)
     b = render.Batch(0)
     b.set(ctxs, passes)
@@ -202,6 +210,8 @@ def batch_bench(reps):
     n_launches = sum(n for (_, n, _, _) in passes)
     return {"metric": "batched Msamples/s", "value": round(samples / t_bat / 1e6, 3), "unit": "Msamples/s",
             "t_batch_ms": round(t_bat * 1e3, 3),
+            "t_plain_ms": round(t_plain * 1e3, 3), "plain_value": round(samples / t_plain / 1e6, 3),
+            "speedup_vs_plain": round(t_plain / t_bat, 3),
             "t_per_pass_ms": round(t_pass * 1e3, 3), "per_pass_launches": n_launches,
             "per_pass_value": round(samples / t_pass / 1e6, 3),
             "speedup_vs_per_pass": round(t_pass / t_bat, 3),
@@ -212,8 +222,9 @@ def batch_bench(reps):
                       "tuning_s": round(t_tune, 1)},
             "workload": f"64 Cornell-box scenes, {tris:,} triangles in total (1k-2M each, log-uniform), "
                         f"128^2-1024^2, spp 4-64, depth 3-10, {samples / 1e6:.1f} M samples per launch; "
-                        "frames bit-identical between the three forms; per_pass = one launch per scene "
-                        "and pass (the reference's dispatch pattern), sequential = one launch per scene"}
+                        "frames bit-identical between the four forms; plain = one launch per scene of the "
+                        "one-thread-per-pixel kernel, per_pass = one persistent launch per scene and pass "
+                        "(the reference's dispatch pattern), sequential = one persistent launch per scene"}
 
 
 def bvh_bench(reps):
@@ -389,8 +400,9 @@ def main():
     batch = None
     if not args.no_batch and args.workload == "c2":
         batch = batch_bench(max(args.steps, 3))
-        print(f"[bench] batch {batch['value']} Msamples/s, {batch['speedup_vs_per_pass']}x per-pass launches, "
-              f"{batch['speedup_vs_sequential']}x per-scene launches",
+        print(f"[bench] batch {batch['value']} Msamples/s, {batch['speedup_vs_plain']}x plain per-scene launches, "
+              f"{batch['speedup_vs_per_pass']}x per-pass launches, "
+              f"{batch['speedup_vs_sequential']}x per-scene persistent launches",
               file=sys.stderr, flush=True)
 
     bvh = None

@@ -1,6 +1,7 @@
 """GPU parity of the batched persistent launch: one graph launch over several contexts
 writes into every frame exactly (bit for bit) what zgl_render of the same passes writes,
-whatever the scene order, regeneration threshold and blocks per CU, and after tuning."""
+whatever the scene order, regeneration threshold and blocks per CU, and after tuning;
+so do the plain and per-pass forms it is timed against."""
 import numpy as np
 import pytest
 
@@ -70,6 +71,22 @@ def test_batch_equals_per_scene_launches(contexts, batch_scenes):
         for k, (f, r) in enumerate(zip(frames, refs)):
             assert np.array_equal(f.view(np.uint32), r.view(np.uint32)), (th, bpc, order, k)
     b.close()
+
+
+def test_plain_and_per_pass_forms_equal_per_scene_launches(contexts, batch_scenes):
+    """The other forms the batched launch is timed against write the same frames: the
+    plain one-thread-per-pixel kernel (zgl_render_replay at the uploaded parameters) and
+    one zgl_render per pass."""
+    print("[gpu] batch: plain and per-pass forms vs zgl_render", flush=True)
+    refs = _reference(contexts, batch_scenes)
+    for k, (c, (ps, spp, depth, first)) in enumerate(zip(contexts, batch_scenes)):
+        c.clear()
+        c.render_replay(first, spp, 1 + first, depth)
+        assert np.array_equal(c.read_frame().view(np.uint32), refs[k].view(np.uint32)), k
+        c.clear()
+        for j in range(spp):
+            c.render(_lib.INTEGRATOR_NAIVE, first + j, 1, 1 + first + j, depth)
+        assert np.array_equal(c.read_frame().view(np.uint32), refs[k].view(np.uint32)), k
 
 
 def test_batch_tune_keeps_results(contexts, batch_scenes):
