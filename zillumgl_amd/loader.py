@@ -67,8 +67,9 @@ def scale(m, v):
 
 
 def rotate(m, angle, axis):
-    """glm::rotate(m, angle, axis) = m * R(angle, normalize(axis))."""
-    c, s = F(np.cos(F(angle))), F(np.sin(F(angle)))
+    """glm::rotate(m, angle, axis) = m * R(angle, normalize(axis)); cos and sin are the
+    correctly rounded FP32 values of the FP64 functions."""
+    c, s = F(math.cos(float(F(angle)))), F(math.sin(float(F(angle))))
     a = normalize3(axis)
     t = (F(1) - c) * a
     R = [[F(c + t[0] * a[0]), F(t[0] * a[1] + s * a[2]), F(t[0] * a[2] - s * a[1])],
@@ -270,27 +271,33 @@ def read_obj(path):
 
 def _smooth_normals(pos, polys, nrm, missing):
     """Per-corner normal = normalised sum of the unit normals (Newell) of the polygons of
-    the mesh that touch its position, each polygon counted once."""
-    p64 = pos.astype(np.float64)
+    the mesh that touch its position, each polygon counted once.  FP64, sums left to
+    right, so that the C++ host reproduces it bit for bit."""
+    p64 = pos.astype(np.float64).tolist()
     keys = {}
-    slot = np.empty(len(pos), np.int64)
-    for i, p in enumerate(map(tuple, pos.tolist())):
-        slot[i] = keys.setdefault(p, len(keys))
-    acc = np.zeros((len(keys), 3))
+    slot = []
+    for p in map(tuple, pos.tolist()):
+        slot.append(keys.setdefault(p, len(keys)))
+    acc = [[0.0, 0.0, 0.0] for _ in range(len(keys))]
     for ids in polys:
-        q = p64[ids]
-        r = np.roll(q, -1, axis=0)
-        n = np.array([np.sum((q[:, 1] - r[:, 1]) * (q[:, 2] + r[:, 2])),
-                      np.sum((q[:, 2] - r[:, 2]) * (q[:, 0] + r[:, 0])),
-                      np.sum((q[:, 0] - r[:, 0]) * (q[:, 1] + r[:, 1]))])
-        ln = np.linalg.norm(n)
+        nx = ny = nz = 0.0
+        k = len(ids)
+        for i in range(k):
+            a, b = p64[ids[i]], p64[ids[(i + 1) % k]]
+            nx += (a[1] - b[1]) * (a[2] + b[2])
+            ny += (a[2] - b[2]) * (a[0] + b[0])
+            nz += (a[0] - b[0]) * (a[1] + b[1])
+        ln = math.sqrt((nx * nx + ny * ny) + nz * nz)
         if ln > 0:
-            for k in set(slot[ids].tolist()):
-                acc[k] += n / ln
-    ln = np.linalg.norm(acc, axis=1)
-    acc = np.where(ln[:, None] > 0, acc / np.maximum(ln, 1e-300)[:, None], 0.0)
+            for j in sorted(set(slot[i] for i in ids)):
+                acc[j][0] += nx / ln
+                acc[j][1] += ny / ln
+                acc[j][2] += nz / ln
     out = nrm.copy()
-    out[missing] = acc[slot[missing]].astype(F)
+    for i in np.nonzero(missing)[0]:
+        a = acc[slot[i]]
+        ln = math.sqrt((a[0] * a[0] + a[1] * a[1]) + a[2] * a[2])
+        out[i] = (a[0] / ln, a[1] / ln, a[2] / ln) if ln > 0 else (0.0, 0.0, 0.0)
     return out
 
 

@@ -132,41 +132,78 @@ def _f(x):
     return np.float32(x)
 
 
+# Every FP32 operation below is one IEEE single operation in a fixed order (no BLAS
+# dot products), and sin, cos and tan are the correctly rounded FP32 values of the FP64
+# functions, so that a C++ host (zillumgl_amd/host/zgl_host.cpp) reproduces the uniforms
+# bit for bit.
+def _trig(fn, x):
+    return _f(fn(float(_f(x))))
+
+
 def _normalize(v):
+    """glm::normalize: v * (1 / sqrt(x*x + y*y + z*z))."""
     v = np.asarray(v, np.float32)
-    return (v * _f(_f(1.0) / np.sqrt(_f(np.dot(v, v))))).astype(np.float32)
+    d = _f(_f(v[0] * v[0]) + _f(v[1] * v[1]))
+    d = _f(d + _f(v[2] * v[2]))
+    return (v * _f(_f(1.0) / _f(np.sqrt(d)))).astype(np.float32)
+
+
+def _cross(a, b):
+    a, b = np.asarray(a, np.float32), np.asarray(b, np.float32)
+    return np.array([_f(a[1] * b[2]) - _f(a[2] * b[1]), _f(a[2] * b[0]) - _f(a[0] * b[2]),
+                     _f(a[0] * b[1]) - _f(a[1] * b[0])], np.float32)
+
+
+def _inverse3(c0, c1, c2):
+    """glm::inverse of the mat3 with columns c0, c1, c2 (cofactors over the determinant
+    expanded along column 0), FP32; returns the inverse's columns."""
+    m = [np.asarray(c, np.float32) for c in (c0, c1, c2)]
+    g = lambda c, r: m[c][r]
+    det = (g(0, 0) * (g(1, 1) * g(2, 2) - g(2, 1) * g(1, 2))
+           - g(1, 0) * (g(0, 1) * g(2, 2) - g(2, 1) * g(0, 2))) \
+        + g(2, 0) * (g(0, 1) * g(1, 2) - g(1, 1) * g(0, 2))
+    o = _f(_f(1.0) / _f(det))
+    inv = [[(g(1, 1) * g(2, 2) - g(2, 1) * g(1, 2)) * o,
+            -(g(0, 1) * g(2, 2) - g(2, 1) * g(0, 2)) * o,
+            (g(0, 1) * g(1, 2) - g(1, 1) * g(0, 2)) * o],
+           [-(g(1, 0) * g(2, 2) - g(2, 0) * g(1, 2)) * o,
+            (g(0, 0) * g(2, 2) - g(2, 0) * g(0, 2)) * o,
+            -(g(0, 0) * g(1, 2) - g(1, 0) * g(0, 2)) * o],
+           [(g(1, 0) * g(2, 1) - g(2, 0) * g(1, 1)) * o,
+            -(g(0, 0) * g(2, 1) - g(2, 0) * g(0, 1)) * o,
+            (g(0, 0) * g(1, 1) - g(1, 0) * g(0, 1)) * o]]
+    return [np.array(col, np.float32) for col in inv]
 
 
 def camera_uniforms(cam: CameraDesc, width, height):
     rad = lambda d: _f(d) * _f(0.01745329251994329576923690768489)
     ax, ay = rad(cam.angle[0]), rad(cam.angle[1])
-    x = _f(np.sin(ax)) * _f(np.cos(ay))
-    y = _f(np.cos(ax)) * _f(np.cos(ay))
-    z = _f(np.sin(ay))
+    x = _trig(math.sin, ax) * _trig(math.cos, ay)
+    y = _trig(math.cos, ax) * _trig(math.cos, ay)
+    z = _trig(math.sin, ay)
     front = _normalize(np.array([x, y, z], np.float32))
-    right = _normalize(np.cross(front, np.array([0, 0, 1], np.float32)).astype(np.float32))
+    right = _normalize(_cross(front, np.array([0, 0, 1], np.float32)))
     roll = _f(cam.angle[2])
     if roll != 0:
         # glm::rotate(mat4(1), roll, front) applied to right (Camera.cpp:159-160)
-        c, s = _f(np.cos(roll)), _f(np.sin(roll))
+        c, s = _trig(math.cos, roll), _trig(math.sin, roll)
         a = _normalize(front)
         t = (_f(1) - c) * a
         Rm = np.array([[c + t[0] * a[0], t[0] * a[1] + s * a[2], t[0] * a[2] - s * a[1]],
                        [t[1] * a[0] - s * a[2], c + t[1] * a[1], t[1] * a[2] + s * a[0]],
                        [t[2] * a[0] + s * a[1], t[2] * a[1] - s * a[0], c + t[2] * a[2]]], np.float32)
         right = _normalize((Rm[0] * right[0] + Rm[1] * right[1] + Rm[2] * right[2]).astype(np.float32))
-    up = _normalize(np.cross(right, front).astype(np.float32))
-    # inverse(mat3(right, up, front)), column-major
-    Mcols = np.stack([right, up, front], axis=1).astype(np.float64)
-    inv = np.linalg.inv(Mcols).astype(np.float32)
+    up = _normalize(_cross(right, front))
+    # glm::inverse(mat3(right, up, front)) (NaivePath.cpp:53-54), column-major
+    inv_cols = _inverse3(right, up, front)
     fov = min(max(cam.fov, 0.1), 90.0)
     cam_struct = _lib.Camera()
     cam_struct.front[:] = front.tolist()
     cam_struct.right[:] = right.tolist()
     cam_struct.up[:] = up.tolist()
     cam_struct.pos[:] = [float(v) for v in np.asarray(cam.pos, np.float32)]
-    cam_struct.mat_inv[:] = inv.T.reshape(-1).tolist()   # column-major
-    cam_struct.tan_fov = float(_f(np.tan(rad(_f(fov) * _f(0.5)))))
+    cam_struct.mat_inv[:] = [float(v) for col in inv_cols for v in col]   # column-major
+    cam_struct.tan_fov = float(_trig(math.tan, rad(_f(fov) * _f(0.5))))
     cam_struct.aspect = float(_f(width) / _f(height))
     cam_struct.lens_radius = float(cam.lens_radius)
     cam_struct.focal_dist = float(cam.focal_dist)
