@@ -19,6 +19,8 @@ import time
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libzgl.so")
+HOST = os.path.join(HERE, "host")
+HOST_OUT = os.path.join(HERE, "libzgl_host.so")
 BUILD = os.path.join(HERE, "_build")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
@@ -69,7 +71,24 @@ def build(verbose=True):
     os.replace(tmp, OUT)
     if verbose:
         print("built", OUT)
+    build_host(verbose)
     return OUT
+
+
+def build_host(verbose=True):
+    """libzgl_host.so: the C++ host layer (zillumgl_amd/host), linked against libzgl.so
+    and found next to it at run time."""
+    objs = []
+    for src in ("zgl_host.cpp", "zgl_host_capi.cpp"):
+        obj = os.path.join(BUILD, src + ".o")
+        _run(["g++", *CXX_FLAGS, "-c", os.path.join(HOST, src), "-o", obj])
+        objs.append(obj)
+    tmp = HOST_OUT + ".tmp"
+    _run(["g++", "-shared", *objs, "-o", tmp, f"-L{HERE}", "-l:libzgl.so", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, HOST_OUT)
+    if verbose:
+        print("built", HOST_OUT)
+    return HOST_OUT
 
 
 if __name__ == "__main__":
