@@ -227,6 +227,43 @@ def batch_bench(reps):
                         "(the reference's dispatch pattern), sequential = one persistent launch per scene"}
 
 
+def integrators_bench(hip, reps):
+    """The four camera/light integrators on the caustics workload (Cornell box + 4 small
+    emissive triangles + a 10k-triangle IOR-1.5 sphere, 1024 x 1024, depth 8): one call
+    of 16 passes each, median of reps.  Naive and global-queue tracers count camera
+    samples; the light tracer counts light paths (64 blocks x 1536 per pass, the
+    reference's LightPathIntegrator dispatch shape); the triple tracer counts camera
+    samples, and runs 64 x 1536 light paths per pass besides."""
+    from zillumgl_amd import scene, render, _lib
+    ps = scene.pack(scene.config3())
+    ctx = render.Context(0)
+    ctx.upload(ps)
+    st = ctx.stream()
+    W, H, passes, depth, blocks = ps.width, ps.height, 16, 8, 64
+    out = {"workload": f"Cornell box + 4 emissive triangles + 10k-triangle dielectric sphere, "
+                       f"{int(ps.indices.shape[0]):,} tris, {W}x{H}, {passes} passes per call, depth {depth}"}
+    for name, integ, unit_count in (("naive", _lib.INTEGRATOR_NAIVE, W * H),
+                                    ("global_queue", _lib.INTEGRATOR_GLOBAL_QUEUE, W * H),
+                                    ("light", _lib.INTEGRATOR_LIGHT, blocks * 1536),
+                                    ("triple", _lib.INTEGRATOR_TRIPLE, W * H)):
+        ms = []
+        for r in range(reps + 1):
+            ctx.clear()
+            a, b = hip.event(), hip.event()
+            hip.record(a, st)
+            ctx.render(integ, 0, passes, 1, depth, blocks_one_pass=blocks, loops_per_pass=1)
+            hip.record(b, st)
+            t = hip.elapsed_ms(a, b)
+            if r > 0:
+                ms.append(t)
+        t = sorted(ms)[len(ms) // 2]
+        key = "light_Mpaths_s" if integ == _lib.INTEGRATOR_LIGHT else f"{name}_Msamples_s"
+        out[key] = round(unit_count * passes / t / 1e3, 2)
+        out[f"{name}_ms"] = round(t, 3)
+    ctx.close()
+    return out
+
+
 def bvh_bench(reps):
     """GPU MTBVH build (zgl_bvh_builder_build) of the largest scene of the batched workload
     (Cornell box tessellated to 1,319,508 triangles), wall clock per build including its
@@ -301,6 +338,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-bvh", action="store_true")
+    ap.add_argument("--no-integrators", action="store_true")
     ap.add_argument("--bvh-only", action="store_true", help="only the MTBVH build line (profiling)")
     ap.add_argument("--reference-spp", type=int, default=0, metavar="N",
                     help="also report the RMSE against an N-spp converged image (N a multiple of 64)")
@@ -411,6 +449,11 @@ def main():
         print(f"[bench] MTBVH build {bvh['device_ms']} ms ({bvh['value']} Mtriangles/s), host {bvh['host_ms']} ms, "
               f"identical {bvh['identical']}", file=sys.stderr, flush=True)
 
+    integ = None
+    if not args.no_integrators and args.workload == "c2":
+        integ = integrators_bench(hip, max(args.steps, 3))
+        print(f"[bench] integrators (caustics workload): {integ}", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(ps, depth)
@@ -430,6 +473,7 @@ def main():
         "adjoint": adjoint,
         "batch": batch,
         "bvh_build": bvh,
+        "integrators": integ,
     }
     if rmse:
         out["rmse"] = rmse
