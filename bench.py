@@ -426,7 +426,7 @@ def main():
                     "rows_per_traversal": round(float(cnt[1]) / max(float(cnt[3]), 1.0), 2),
                     "leaf_tests_per_traversal": round(float(cnt[2]) / max(float(cnt[3]), 1.0), 2),
                     "traversals_per_sample": round(float(cnt[3]) / samples, 3),
-                    "kernel": "k_naive_persistent<false> (zillumgl_amd/csrc/zgl_persistent.hip)"}
+                    "kernel": "k_naive_persistent<false, false> (zillumgl_amd/csrc/zgl_persistent.hip)"}
     ctx.close()
 
     adjoint = None

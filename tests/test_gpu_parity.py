@@ -220,6 +220,31 @@ def test_triple_matches_oracle(ctx, c3_small):
     assert err[sig].max() <= 1e-4
 
 
+@pytest.mark.parametrize("threshold", [1, 20, 63])
+def test_triple_persistent_thresholds_match_oracle(ctx, c3_small, threshold):
+    """The triple tracer's camera pass on the persistent schedule, passes 2..4 (free
+    counter 3) in one call, at several regeneration thresholds; and the one-thread-per-
+    pixel camera pass (the byte-counting variant) gives the same film."""
+    print(f"[gpu] triple persistent threshold {threshold}", flush=True)
+    ctx.upload(c3_small)
+    ctx.clear()
+    ctx.render(_lib.INTEGRATOR_TRIPLE, 2, 3, 3, 8, blocks_one_pass=8, regen_threshold=threshold)
+    gpu = ctx.read_film()
+    prm = orc.params(2, 2, 3, 3, 8, blocks_one_pass=8)
+    pt, _ = orc.render_pixels(c3_small, prm)
+    lpt, _ = orc.render_light(c3_small, prm)
+    ref = pt.astype(np.float64) + lpt
+    floor = 1e-3 * max(ref.mean(), 1e-12)
+    sig = ref > floor
+    err = _rel_err(gpu, ref, floor)
+    print(f"[gpu] triple persistent: max rel err {err[sig].max():.3g}", flush=True)
+    assert err[sig].max() <= 1e-4
+    ctx.clear()
+    ctx.render(_lib.INTEGRATOR_TRIPLE, 2, 3, 3, 8, blocks_one_pass=8, count_bytes=True)
+    plain = ctx.read_film()
+    assert _rel_err(plain, gpu.astype(np.float64), floor)[sig].max() <= 1e-4
+
+
 @pytest.mark.parametrize("threshold", [1, 16, 40, 63])
 def test_persistent_thresholds_match_oracle(ctx, c2_small, threshold):
     """Whatever the regeneration threshold, each walk visits the same rows and each

@@ -92,6 +92,7 @@ struct zgl_context {
     std::vector<float> hostMaterials;  // uploaded material records (16 floats each)
     int numCUs = 0;
     int naiveBlocksPerCU = 0;
+    int tripleBlocksPerCU = 0;
     DevScene scene = {};
     bool hasScene = false, hasCamera = false, hasNoise = false;
     int filmW = 0, filmH = 0;
@@ -641,7 +642,21 @@ zgl_status zgl_render(zgl_context* ctx, const zgl_pass_params* p)
         } else {
             // uScale = W*H / (blocks * loops * 1536)  (TriplePath.cpp:76-77)
             pp.lptScale = (float)(ctx->filmW * ctx->filmH) / (float)(p->blocks_one_pass * pp.loopsPerPass * 1536);
-            hipCheck(launchTriplePT(s, pp, film, cnt, ctx->stream), "triple camera pass");
+            if (cnt) {
+                // the byte-counting variant of the camera pass: one thread per pixel
+                hipCheck(launchTriplePT(s, pp, film, cnt, ctx->stream), "triple camera pass");
+            } else {
+                if (!ctx->tripleBlocksPerCU) ctx->tripleBlocksPerCU = triplePersistentBlocksPerCU();
+                const size_t perPass = (size_t)ctx->filmW * ctx->filmH * 16;
+                const int resPasses =
+                    (int)std::max<size_t>(1, std::min<size_t>(pp.nPasses, ((size_t)4 << 30) / perPass));
+                ctx->ensure(ctx->naiveRes, perPass * resPasses, "sample values");
+                const int threshold = p->regen_threshold > 0 ? p->regen_threshold : kDefaultRegenThreshold;
+                hipCheck(launchTriplePersistent(s, pp, film, (float4*)ctx->naiveRes.p, resPasses,
+                                                (int*)ctx->workCounter.p, threshold,
+                                                ctx->numCUs * ctx->tripleBlocksPerCU, ctx->stream),
+                         "triple camera pass");
+            }
             DevScene sl = s;
             sl.sampler = 0;  // TriplePath.cpp:72
             hipCheck(launchTripleLPT(sl, pp, film, cnt, ctx->stream), "triple light pass");

@@ -304,6 +304,9 @@ ZGL_D void triangleVerts(const DevScene& s, int id, vec3& a, vec3& b, vec3& c, C
     a = ldV3(s.vertices, ia); b = ldV3(s.vertices, ib); c = ldV3(s.vertices, ic);
     cnt<C>(cn, 48);
 }
+// The triple tracer's pdf remap: p^2, or 1 for p < 1e-8 (triple_path_pass_pt.glsl).
+ZGL_D float remapP(float p) { return p < 1e-8f ? 1.0f : p * p; }
+
 template <bool C> ZGL_D float triangleAreaId(const DevScene& s, int id, Counters& cn)
 {
     vec3 a, b, c;

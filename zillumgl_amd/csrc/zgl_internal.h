@@ -52,6 +52,11 @@ hipError_t launchNaivePersistent(const DevScene& s, const PassParams& p, float4*
                                  int* pixelCounter, int threshold, int blocks, unsigned long long* counters,
                                  hipStream_t st);
 int naivePersistentBlocksPerCU();
+// The triple tracer's camera pass on the persistent schedule: p.nPasses passes in chunks
+// of resPasses (res holds resPasses x W x H values), added to the planar film in order.
+hipError_t launchTriplePersistent(const DevScene& s, const PassParams& p, float* film, float4* res, int resPasses,
+                                  int* pixelCounter, int threshold, int blocks, hipStream_t st);
+int triplePersistentBlocksPerCU();
 // Traversal results logged per sample for the adjoint (see launchNaiveLogged).
 constexpr int kHitLogQueries = 16;
 // The persistent naive tracer over p's passes (all of them in one launch: res holds

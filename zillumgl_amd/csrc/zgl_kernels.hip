@@ -257,7 +257,6 @@ __global__ __launch_bounds__(kBlock) void k_light(DevScene s, PassParams prm, fl
 // Triple path tracer, camera pass (triple_path_pass_pt.glsl:44-195): s=0 and s=1
 // strategies weighted against t=1 with the p^2 heuristic.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float remapP(float p) { return p < 1e-8f ? 1.0f : p * p; }
 
 template <bool C>
 __device__ vec3 tripleCameraTrace(const DevScene& s, const PassParams& prm, Ray ray, Rng& rng, Counters& cn)

@@ -475,6 +475,297 @@ __global__ __launch_bounds__(256, 4) void k_naive_batch(const BatchScene* __rest
                                   threshold, nullptr, nullptr);
 }
 
+// ---------------------------------------------------------------------------
+// Triple path tracer, camera pass (triple_path_pass_pt.glsl:44-195) on the same
+// persistent schedule: one sample per (8x8 tile, pass) unit, XCD regions, walks advanced
+// a few rows per ballot round with lanes shading and reissuing rays once no more than
+// `threshold` are still walking.  The shading steps are tripleCameraTrace's, in its order
+// of operations and random-number use; the s=1 term is evaluated before its visibility
+// walk and added if the walk finds no occluder.  Each sample's value goes to
+// res[pass * W * H + pixel]; k_triple_accumulate adds them to the planar film.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256, 4) void k_triple_persistent(DevScene s, PassParams prm, float4* __restrict__ res,
+                                                           int* __restrict__ pixelCounter, int threshold)
+{
+    const int lane = threadIdx.x & 63;
+    Counters cn;
+    int region = (int)(blockIdx.x % kRegions);
+    int exhausted = 0;
+    const int W = s.filmW, H = s.filmH;
+    const int nIndex = ((W + 7) / 8) * ((H + 7) / 8) * 64;
+    const int nUnits = nIndex * prm.nPasses;
+
+    int phase = kNewPixel;
+    int pix = -1, pass = 0;
+    vec2 scr = v2(0.f, 0.f);
+    PixelSeed ps = {0u, 0u};
+    Rng rng = {0u, 0u, 0, 0};
+    vec3 result = v3(0.f), throughput = v3(1.f), wo = v3(0.f), pos = v3(0.f), wi = v3(0.f);
+    vec3 prevNorm = v3(0.f);
+    int bounce = 0, id = -1;
+    float t1s0 = 0.f, t1s1 = 0.f, pdfNext = 0.f, cPrev = 1.f, rrProb = 0.f;
+    bool terminate = false;
+    vec3 pend = v3(0.f);
+    bool walking = false;
+    Walk wk;
+    wk.p.o = v3(0.f); wk.p.d = v3(0.f); wk.p.dInv = v3(0.f); wk.p.mode = 0;
+    wk.p.nx = wk.p.ny = wk.p.nz = false; wk.p.table = 0;
+    wk.dist = 0.f; wk.k = 0; wk.closest = -1; wk.any = false;
+
+    for (;;) {
+        if (!walking) {
+            if (phase == kPrimary) {
+                const int hit = wk.closest;
+                const Ray ray = makeRay(wk.p.o, wk.p.d);
+                const float primDist = wk.dist;
+                pos = rayPoint(ray, primDist);
+                if (hit == -1) {
+                    result = envLe<false>(s, ray.dir, cn);
+                    phase = kFinish;
+                } else if (hit - s.objPrimCount >= 0) {
+                    result = lightLe<false>(s, hit - s.objPrimCount, pos, -ray.dir, cn);
+                    phase = kFinish;
+                } else {
+                    const SurfaceInfo surf = triangleSurfaceInfo<false>(s, hit, pos, cn);
+                    wo = -ray.dir;
+                    prevNorm = s.camF;
+                    result = v3(0.0f);
+                    throughput = v3(1.0f);
+                    const CameraPdf camPdf = cameraPdfIe(s, ray);
+                    const float primaryPdf =
+                        remapP(camPdf.pdfPos) / remapP(camPdf.pdfDir * absDot(surf.ns, ray.dir) / square(primDist));
+                    t1s0 = primaryPdf;
+                    t1s1 = primaryPdf;
+                    id = hit;
+                    bounce = 1;
+                    phase = (prm.maxDepth >= 1) ? kSurface : kFinish;
+                }
+            } else if (phase == kShadow) {
+                if (wk.closest < 0) result += pend;
+                if (terminate) {
+                    phase = kFinish;
+                } else {
+                    walkStart<false>(s, wk, rayOffseted(pos, wi), false, 1e8f, cn);
+                    walking = true;
+                    phase = kExtend;
+                }
+            } else if (phase == kExtend) {
+                const int nextId = wk.closest;
+                const int lightId = nextId - s.objPrimCount;
+                const vec3 nextPos = rayPoint(makeRay(wk.p.o, wk.p.d), wk.dist);
+                const float dist = wk.dist;
+                if (nextId == -1) {
+                    phase = kFinish;
+                } else if (lightId >= 0) {
+                    const vec3 nLit = triangleSurfaceInfo<false>(s, nextId, nextPos, cn).ng;
+                    const LightPdf pl = lightPdfLe<false>(s, lightId, makeRay(nextPos, -wi), cn);
+                    const float pdfPLit = remapP(pl.pdfPos * lightPdfSampleOne<false>(s, lightId, cn));
+                    const float coefToLight = remapP(pdfNext * satDot(nLit, -wi));
+                    const float coefToSurf = remapP(pl.pdfDir * absDot(prevNorm, wi));
+                    const float coefDist = remapP(dist * dist);
+                    const float s1s0 = pdfPLit * coefDist / coefToLight;
+                    const float t1s0x = t1s0 * coefToSurf * pdfPLit * cPrev / coefToLight;
+                    const float weight = fisnan(t1s0) ? 0.0f : 1.0f / (1.0f + s1s0 + t1s0x);
+                    result += lightLe<false>(s, lightId, nextPos, -wi, cn) * throughput * weight;
+                    phase = kFinish;
+                } else {
+                    phase = kSurface;
+                    if (prm.russianRoulette) {
+                        if (sample1D<false>(s, rng, cn) >= rrProb) phase = kFinish;
+                        else throughput /= rrProb;
+                    }
+                    if (phase == kSurface) {
+                        const float coef = cPrev / remapP(pdfNext * absDot(triangleNormalShad<false>(s, nextId, nextPos, cn), wi));
+                        t1s0 *= coef;
+                        t1s1 *= coef;
+                        pos = nextPos;
+                        wo = -wi;
+                        id = nextId;
+                        bounce++;
+                        if (bounce > prm.maxDepth) phase = kFinish;
+                    }
+                }
+            }
+
+            if (phase == kSurface) {
+                SurfaceInfo surf = triangleSurfaceInfo<false>(s, id, pos, cn);
+                const int matId = s.matTexIndices[id] & 0x0000ffff;
+                const uint32_t matType = loadMaterialType<false>(s, matId, cn);
+                if (matType != MatDielectric && matType != MatThinDielectric) {
+                    if (dot(surf.ns, wo) < 0.0f) flipNormals(surf);
+                }
+                const BSDFParam mp = loadMaterial<false>(s, matType, matId, cn);
+                bool shadow = false;
+                Ray shadowRay;
+                float shadowDist = 0.f;
+                {
+                    const int light = lightSampleOne<false>(s, sample2D<false>(s, rng, cn), cn);
+                    const int triId = light + s.objPrimCount;
+                    const float pdfSource = lightPdfSampleOne<false>(s, light, cn);
+                    const vec3 pLit = triangleSampleUniformId<false>(s, triId, sample2D<false>(s, rng, cn), cn);
+                    const vec3 wl = normalize(pLit - pos);
+                    const vec3 Le = lightLe<false>(s, light, pLit, -wl, cn);
+                    if (!isBlack(Le)) {
+                        // visible(pos, pLit): offset 1e-5 along the direction, 2e-5 short of pLit
+                        shadow = true;
+                        shadowDist = distance(pos, pLit) - 2e-5f;
+                        const vec3 wv = normalize(pLit - pos);
+                        shadowRay = makeRay(pos + wv * 1e-5f, wv);
+                        const vec3 nLit = triangleSurfaceInfo<false>(s, triId, pLit, cn).ng;
+                        const float pA = pdfSource / triangleAreaId<false>(s, triId, cn);
+                        const float dist2 = distSquare(pos, pLit);
+                        const float pS = pA * dist2 / absDot(nLit, wl);
+                        const vec3 f = materialBSDF(matType, mp, wo, wl, surf.ns, ModeRadiance);
+                        const float fpdf = materialPdf(matType, mp, wo, wl, surf.ns);
+                        const float pdfRev = materialPdf(matType, mp, wl, wo, surf.ns);
+                        const float pdfPLit = remapP(pA);
+                        const float coefToSurf = remapP(0.5f * PiInv * absDot(surf.ns, wl));
+                        const float coefToLight = remapP(fpdf * satDot(nLit, -wl));
+                        const float coefToPrev = (bounce == 1) ? 1.0f : remapP(pdfRev * absDot(prevNorm, wo));
+                        const float coefDist = remapP(dist2);
+                        const float s1s0 = pdfPLit * coefDist / coefToLight;
+                        const float t1s1x = t1s1 * coefToSurf * coefToPrev / coefDist;
+                        const float weight = s1s0 / (1.0f + s1s0 + s1s0 * t1s1x);
+                        pend = Le * f * throughput * absDot(surf.ns, wl) / pS * weight;
+                    }
+                }
+                const BSDFSample bs = materialSample(matType, mp, surf.ns, wo, ModeRadiance, sample3D<false>(s, rng, cn), rng);
+                wi = bs.wi;
+                const float bsdfPdf = bs.pdf;
+                const bool deltaBsdf = (bs.flag == FlagSpecRefl || bs.flag == FlagSpecTrans);
+                terminate = bsdfPdf < 1e-8f;
+                if (!terminate) {
+                    throughput *= bs.bsdf / bsdfPdf * (deltaBsdf ? 1.0f : absDot(surf.ns, wi));
+                    pdfNext = materialPdf(matType, mp, wo, wi, surf.ns);
+                    const float pdfPrev = materialPdf(matType, mp, wi, wo, surf.ns);
+                    cPrev = (bounce == 1) ? 1.0f : remapP(pdfPrev * absDot(prevNorm, wo));
+                    rrProb = fminG(maxComponent(bs.bsdf / bsdfPdf), 0.95f);
+                    prevNorm = surf.ns;
+                }
+                if (shadow) {
+                    walkStart<false>(s, wk, shadowRay, true, shadowDist, cn);
+                    walking = true;
+                    phase = kShadow;
+                } else if (terminate) {
+                    phase = kFinish;
+                } else {
+                    walkStart<false>(s, wk, rayOffseted(pos, wi), false, 1e8f, cn);
+                    walking = true;
+                    phase = kExtend;
+                }
+            }
+
+            if (phase == kFinish) {
+                res[(size_t)pass * ((size_t)W * H) + pix] = make_float4(result.x, result.y, result.z, 0.0f);
+                phase = kNewPixel;
+            }
+        }
+
+        // samples are handed out per wave, one region of 8x8 tiles per XCD first
+        const unsigned long long need = __ballot(!walking && phase == kNewPixel);
+        if (need) {
+            const int n = __popcll(need);
+            const int leader = __ffsll((long long)need) - 1;
+            int base = 0;
+            const int len = ((nUnits / kRegions + 63) / 64) * 64;
+            const int start = min(nUnits, region * len);
+            const int limit = min(nUnits, start + len);
+            if (lane == leader) base = atomicAdd(pixelCounter + region, n);
+            base = __shfl(base, leader) + start;
+            if (base >= limit) {
+                region = (region + 1) % kRegions;
+                exhausted++;
+            } else {
+                exhausted = 0;
+            }
+            if (!walking && phase == kNewPixel) {
+                const unsigned long long below = need & ((1ull << lane) - 1ull);
+                const int idx = base + __popcll(below);
+                pix = -1;
+                if (exhausted >= kRegions) {
+                    phase = kDone;
+                } else if (idx < limit) {
+                    const int group = idx >> 6;
+                    const int unitPass = group % prm.nPasses;
+                    const int pixIdx = (group / prm.nPasses) * 64 + (idx & 63);
+                    int x, y;
+                    pixelOfIndex(pixIdx, W, x, y);
+                    if (x < W && y < H) {
+                        pix = y * W + x;
+                        scr = v2((float)x, (float)y) / v2((float)W, (float)H);
+                        ps = pixelSeed(s, scr);
+                        pass = unitPass;
+                        phase = kNewSample;
+                    }
+                }
+            }
+        }
+
+        if (!walking && phase == kNewSample) {
+            const int spp = prm.firstPass + pass;
+            const uint32_t fc = (uint32_t)(prm.freeCounter + pass);
+            rng.randSeed = ps.tx * fc + ps.ty;
+            rng.sampleSeed = ps.tx * ps.ty;
+            rng.sampleOffset = sampleOffsetOf(spp);
+            rng.dim = 0;
+            const vec4 u = sample4D<false>(s, rng, cn);
+            walkStart<false>(s, wk, cameraSampleRay(s, scr, u), false, 1e8f, cn);
+            walking = true;
+            phase = kPrimary;
+        }
+
+        if (__ballot(phase != kDone) == 0ull) break;
+
+        {
+            bool general = __ballot(walking && wk.p.special) == 0ull;
+            for (;;) {
+#pragma unroll
+                for (int u = 0; u < kRowsPerRound; ++u) {
+                    if (walking && wk.leaf < 0) {
+                        if (walkRow<false>(s, wk, general, cn)) walking = false;
+                    }
+                }
+                if (walking && wk.leaf >= 0) {
+                    if (walkLeaf<false>(s, wk, cn)) walking = false;
+                }
+                // a visibility walk that ended settles its s=1 term and starts the extension
+                if (!walking && phase == kShadow) {
+                    if (wk.closest < 0) result += pend;
+                    if (terminate) {
+                        phase = kFinish;
+                    } else {
+                        walkStart<false>(s, wk, rayOffseted(pos, wi), false, 1e8f, cn);
+                        walking = true;
+                        phase = kExtend;
+                    }
+                }
+                general = __ballot(walking && wk.p.special) == 0ull;
+                if (__popcll(__ballot(walking)) <= threshold) break;
+            }
+        }
+    }
+}
+
+// The camera pass's samples into the planar RGB film (3 floats per pixel, rows of 3W),
+// pass by pass in order, NaN samples skipped (triple_path_pass_pt.glsl:218-222).
+__global__ __launch_bounds__(256) void k_triple_accumulate(const float4* __restrict__ res, int W, int H, int nPasses,
+                                                         float* __restrict__ film)
+{
+    const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = W * H;
+    if (pix >= n) return;
+    float* f = film + (size_t)(pix / W) * 3 * W + 3 * (pix % W);
+    vec3 acc = v3(f[0], f[1], f[2]);
+    for (int k = 0; k < nPasses; k++) {
+        const float4 r = res[(size_t)k * n + pix];
+        const vec3 v = v3(r.x, r.y, r.z);
+        if (!hasNan(v)) acc = acc + v;
+    }
+    f[0] = acc.x;
+    f[1] = acc.y;
+    f[2] = acc.z;
+}
+
 // Zeroes the per-scene pixel counters of a batch (a kernel, so that it can sit in a graph).
 __global__ void k_reset_counters(int* c, int n)
 {
@@ -533,6 +824,32 @@ hipError_t launchNaiveBatch(const BatchScene* items, int nScenes, int maxPixels,
     hipLaunchKernelGGL(k_accumulate_batch, dim3((unsigned)((maxPixels + 255) / 256), (unsigned)std::min(nScenes, 4096)),
                        dim3(256), 0, st, items, nScenes);
     return hipGetLastError();
+}
+
+hipError_t launchTriplePersistent(const DevScene& s, const PassParams& p, float* film, float4* res, int resPasses,
+                                  int* pixelCounter, int threshold, int blocks, hipStream_t st)
+{
+    if (threshold < 0) threshold = 0;
+    if (threshold > 63) threshold = 63;
+    const int n = s.filmW * s.filmH;
+    for (int k0 = 0; k0 < p.nPasses; k0 += resPasses) {
+        PassParams q = p;
+        q.firstPass = p.firstPass + k0;
+        q.freeCounter = p.freeCounter + k0;
+        q.nPasses = std::min(resPasses, p.nPasses - k0);
+        hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, pixelCounter);
+        hipLaunchKernelGGL(k_triple_persistent, dim3(blocks), dim3(256), 0, st, s, q, res, pixelCounter, threshold);
+        hipLaunchKernelGGL(k_triple_accumulate, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, res, s.filmW,
+                           s.filmH, q.nPasses, film);
+    }
+    return hipGetLastError();
+}
+
+int triplePersistentBlocksPerCU()
+{
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_triple_persistent, 256, 0) != hipSuccess) return 1;
+    return n > 0 ? n : 1;
 }
 
 int naiveBatchBlocksPerCU()
