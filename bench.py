@@ -317,6 +317,9 @@ def rmse_vs_converged(ctx, W, H, spp, depth, ref_spp):
         ctx.clear()
         ctx.render(_lib.INTEGRATOR_NAIVE, first, chunk, 1 + first, depth)
         acc += ctx.read_frame()[..., :3]
+        done = first + chunk - spp
+        if done % (chunk * 128) == 0:
+            print(f"[bench] reference passes {done}/{ref_spp}", file=sys.stderr, flush=True)
     ref = acc / ref_spp
     mse = float(((img - ref) ** 2).mean())
     mse_corrected = mse / (1.0 + spp / ref_spp)
