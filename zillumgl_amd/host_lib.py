@@ -105,7 +105,9 @@ class HostScene:
 
     def upload(self, ctx):
         check(lib().zglh_upload(ctx.h, self.h))
-        ctx.width, ctx.height = self.arrays()[1].film_width, self.arrays()[1].film_height
+        w, h = C.c_int32(), C.c_int32()
+        check(lib().zglh_scene_buffers(self.h, None, None, None, C.byref(w), C.byref(h)))
+        ctx.width, ctx.height = w.value, h.value
 
 
 class HostIntegrator:
