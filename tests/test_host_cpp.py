@@ -199,6 +199,38 @@ def test_generated_scenes_match_python_host(tmp_path, seed):
     assert arr["n_light_triangles"] > 2 and arr["materials"].shape[0] == 7 * 2
 
 
+@pytest.mark.parametrize("text", [
+    "<scene><integrator><size width='4' height='4'/></integrator></scen>",          # mismatched end tag
+    "<scene><integrator><size width='4 height='4'/></integrator></scene>",          # unterminated value
+    "<scene a='&bogus;'><integrator><size width='4' height='4'/></integrator></scene>",  # unknown entity
+    "<scene><integrator><size width='4' height='4'/></integrator></scene><extra/>",  # two roots
+    "",                                                                              # empty file
+    "<scene><integrator><size width='4x' height='4'/></integrator></scene>",        # malformed integer
+])
+def test_malformed_xml_is_rejected(tmp_path, text):
+    bad = tmp_path / "bad.xml"
+    bad.write_text(text)
+    with pytest.raises(_lib.ZglError) as e:
+        host_lib.HostScene(str(bad))
+    assert e.value.status == _lib.ZGL_E_INVALID
+
+
+def test_xml_comments_entities_and_declaration(tmp_path):
+    """Comments, a declaration, a DOCTYPE, character references and single quotes parse
+    as ElementTree reads them."""
+    (tmp_path / "quad.obj").write_text(open(os.path.join(SCENES, "quad.obj")).read())
+    xml = ("<?xml version='1.0' encoding='utf-8'?>\n<!DOCTYPE scene>\n<!-- a comment -->\n"
+           "<scene><integrator><size width='&#56;' height=\"6\"/></integrator><!-- inner -->"
+           "<camera><position value='0 -3&#x20;1'/></camera>"
+           "<modelInstances><modelInstance path='quad.obj' type='object'><transform translate='0 0 0' "
+           "scale='1 1 1' rotate='0 0 0'/><material type='lambertian'><baseColor value='0.5 0.5 0.5'/>"
+           "</material></modelInstance></modelInstances></scene>")
+    path = tmp_path / "ok.xml"
+    path.write_text(xml)
+    arr = _compare(str(path))
+    assert (arr["width"], arr["height"]) == (8, 6)
+
+
 def test_errors_are_status_codes(tmp_path):
     bad = tmp_path / "bad.xml"
     bad.write_text("<scene><integrator><size width='4' height='4'/></integrator>")
