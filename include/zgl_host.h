@@ -17,6 +17,8 @@ extern "C" {
 typedef struct zglh_scene zglh_scene;
 typedef struct zglh_integrator zglh_integrator;
 
+/* The calling thread's last failure message (the reference aborts instead,
+ * src/util/Error.h:24-42). */
 const char* zglh_last_error(void);
 
 /* Scene::load (src/core/Scene.cpp:58-127) of an XML scene with OBJ/MTL models and a .hdr
@@ -24,10 +26,13 @@ const char* zglh_last_error(void);
  * device_bvh != 0 leaves the MTBVH to zgl_upload_scene's device build. */
 zgl_status zglh_load_scene(const char* xml_path, int32_t device_bvh, zglh_scene** out);
 void zglh_scene_destroy(zglh_scene* scene);
-/* The packed buffers (owned by the scene), camera uniforms and seed texture. */
+/* The packed buffers (owned by the scene): SceneGLContext (src/core/Scene.h:12-27),
+ * the camera uniforms NaivePathIntegrator::updateUniforms sets (NaivePath.cpp:49-60) and
+ * the seed texture (Sampler.cpp:66-80). */
 zgl_status zglh_scene_buffers(const zglh_scene* scene, zgl_scene_buffers* buffers, zgl_camera* camera,
                               const float** noise_rg, int32_t* width, int32_t* height);
-/* zgl_upload_scene + zgl_set_camera + zgl_set_noise. */
+/* zgl_upload_scene + zgl_set_camera + zgl_set_noise: what Scene::createGLContext and
+ * updateUniforms hand to GL (Scene.cpp:245-258, NaivePath.cpp:12-67). */
 zgl_status zglh_upload(zgl_context* ctx, const zglh_scene* scene);
 
 /* Integrator of type ZGL_INTEGRATOR_NAIVE / _LIGHT / _TRIPLE / _GLOBAL_QUEUE

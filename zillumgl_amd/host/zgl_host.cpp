@@ -16,11 +16,9 @@
 #include <cstring>
 #include <fstream>
 #include <map>
-#include <memory>
 #include <set>
 #include <sstream>
 #include <stdexcept>
-#include <unordered_map>
 
 namespace zgl {
 namespace host {
