@@ -259,6 +259,37 @@ def test_persistent_thresholds_match_oracle(ctx, c2_small, threshold):
     assert np.array_equal(gpu.view(np.uint32), ref.view(np.uint32))
 
 
+@pytest.mark.parametrize("threshold", [1, 20, 63])
+def test_persistent_roulette_is_exact_and_repeatable(ctx, c2_small, threshold):
+    """Russian roulette on the Disney + environment scene at depth 8 (the roulette ends
+    paths at every depth): the frame is the oracle's bit for bit at every regeneration
+    threshold, and a second launch gives the same bits."""
+    print(f"[gpu] persistent roulette threshold {threshold}", flush=True)
+    ctx.upload(c2_small)
+    frames = []
+    for _ in range(2):
+        ctx.clear()
+        ctx.render(_lib.INTEGRATOR_NAIVE, 1, 4, 2, 8, russian_roulette=True, regen_threshold=threshold)
+        frames.append(ctx.read_frame()[..., :3].copy())
+    ref, _ = orc.render_pixels(c2_small, orc.params(0, 1, 4, 2, 8, russian_roulette=True))
+    assert np.array_equal(frames[0].view(np.uint32), ref.view(np.uint32))
+    assert np.array_equal(frames[0].view(np.uint32), frames[1].view(np.uint32))
+
+
+def test_triple_persistent_roulette_matches_plain_camera_pass(ctx, c3_small):
+    """With roulette, the persistent camera pass and the one-thread-per-pixel one (the
+    byte-counting variant) give the same film up to the light pass's atomic order."""
+    ctx.upload(c3_small)
+    films = []
+    for count in (False, True):
+        ctx.clear()
+        ctx.render(_lib.INTEGRATOR_TRIPLE, 0, 4, 1, 8, russian_roulette=True, blocks_one_pass=8, count_bytes=count)
+        films.append(ctx.read_film().astype(np.float64))
+    floor = 1e-3 * max(films[1].mean(), 1e-12)
+    sig = films[1] > floor
+    assert _rel_err(films[0], films[1], floor)[sig].max() <= 1e-4
+
+
 @pytest.mark.parametrize("kw", [dict(max_depth=5, russian_roulette=True), dict(max_depth=0),
                                 dict(max_depth=3, sample_light=False), dict(max_depth=4, light_env_uniform=True,
                                                                             light_portion=0.3)])
