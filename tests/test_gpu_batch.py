@@ -89,6 +89,30 @@ def test_plain_and_per_pass_forms_equal_per_scene_launches(contexts, batch_scene
         assert np.array_equal(c.read_frame().view(np.uint32), refs[k].view(np.uint32)), k
 
 
+def test_batch_with_roulette_and_options_equals_per_scene_launches(contexts, batch_scenes):
+    """Per-scene options in one batch (roulette on some scenes, NEE off or uniform light
+    selection on others): every frame equals zgl_render of the same options bit for bit."""
+    print("[gpu] batch: mixed options incl. roulette", flush=True)
+    opts = [dict(russian_roulette=True), dict(sample_light=False), dict(light_env_uniform=True, light_portion=0.3),
+            dict(russian_roulette=True, sample_light=False), dict(), dict(russian_roulette=True)]
+    params = []
+    refs = []
+    for c, (ps, spp, depth, first), o in zip(contexts, batch_scenes, opts):
+        kw = dict(first_pass=first, n_passes=spp, free_counter=1 + first, max_depth=depth, **o)
+        params.append(kw)
+        c.clear()
+        c.render(_lib.INTEGRATOR_NAIVE, first, spp, 1 + first, depth, **o)
+        refs.append(c.read_frame())
+    b = render.Batch(0)
+    b.set(contexts, params)
+    for c in contexts:
+        c.clear()
+    frames = _batch_frames(b, contexts)
+    for k, (f, r) in enumerate(zip(frames, refs)):
+        assert np.array_equal(f.view(np.uint32), r.view(np.uint32)), k
+    b.close()
+
+
 def test_batch_tune_keeps_results(contexts, batch_scenes):
     print("[gpu] batch: autotuner", flush=True)
     passes = [(0, spp, 1, depth) for (_, spp, depth, _) in batch_scenes]
