@@ -362,7 +362,7 @@ def main():
         print(json.dumps({"copy_probe_bytes": args.copy_probe, "launches": 3}))
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (this build runs on a single MI355X); --gpus {args.gpus} "
+        print(f"bench.py measures one GPU only (the library is measured on a single MI355X); --gpus {args.gpus} "
               f"is not supported", file=sys.stderr)
         sys.exit(2)
 

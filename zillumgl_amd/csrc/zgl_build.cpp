@@ -176,7 +176,7 @@ void buildMtbvh(const float* vertices, const uint32_t* indices, int64_t nTri,
         if (hi == n) hi--;
         // An empty left child needs bucket 0 empty, i.e. no centroid at the node's
         // minimum: only centroids at or beyond the empty box's 1e8 do that.  The
-        // reference would recurse on an empty range; this build rejects the scene.
+        // reference would recurse on an empty range; the library rejects the scene.
         if (hi == 0) throw std::invalid_argument("MTBVH split with an empty child (centroids beyond 1e8)");
         const int split = l + hi - 1;
 

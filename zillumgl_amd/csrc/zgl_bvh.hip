@@ -5,7 +5,7 @@
 // stably (left part in order, right part reversed), and lays the tree out in DFS preorder:
 // node offset o, left child o + 1, right child o + 2 (split - l) + 2.  A node's result
 // depends only on the primitives its range holds when it is processed, so nodes can be
-// processed in any order that respects parent -> child; this build goes level by level.
+// processed in any order that respects parent -> child; these kernels go level by level.
 //
 // Box reductions.  The host folds boxes with minG(x, y) = y < x ? y : x (and maxG alike),
 // starting from {1e8, -1e8}: the result is the leftmost element attaining the minimum,
