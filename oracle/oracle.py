@@ -72,6 +72,12 @@ def lib():
         L.orc_sobol_sample.restype = C.c_uint32
         L.orc_hash.argtypes = [C.c_uint32]
         L.orc_hash.restype = C.c_uint32
+        L.orc_hash_n.argtypes = [_up, _up, C.c_int64]
+        L.orc_hash_n.restype = None
+        L.orc_set_variant.argtypes = [C.c_int]
+        L.orc_set_variant.restype = None
+        L.orc_env_sample.argtypes = [C.POINTER(OrcScene), C.POINTER(OrcParams), _fp, C.c_int64, _fp, _ip]
+        L.orc_env_pdf.argtypes = [C.POINTER(OrcScene), C.POINTER(OrcParams), _fp, C.c_int64, _fp]
         L.orc_fmath.argtypes = [C.c_int, _fp, _fp, _fp, C.c_int64]
         L.orc_fmath.restype = None
         L.orc_bsdf_eval_f64.argtypes = [_dp, C.c_int, _dp, _dp, _dp, _dp]
@@ -141,6 +147,56 @@ def sobol_matrices():
 
 def hash_u32(x):
     return int(lib().orc_hash(int(x) & 0xFFFFFFFF))
+
+
+def hash_many(x):
+    a = np.ascontiguousarray(x, np.uint32)
+    out = np.empty_like(a)
+    lib().orc_hash_n(_p(a, C.c_uint32), _p(out, C.c_uint32), a.size)
+    return out
+
+
+# Estimator variants (bit flags of orc_set_variant), for tests that pin the restatement.
+VAR_EMIT_COS = 1          # triple camera pass: the light's actual cos/pi emission pdf in its weights
+VAR_PLANT_PREV = 2        # planted mistake: the s=1 weight drops its reverse-pdf factor
+VAR_PLANT_ENV_SIN = 4     # planted mistake: env pdf divided by sin(theta)
+
+
+class variant:
+    """Context manager that runs the oracle with estimator variant flags set."""
+
+    def __init__(self, flags):
+        self.flags = flags
+
+    def __enter__(self):
+        lib().orc_set_variant(self.flags)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_variant(0)
+        return False
+
+
+def env_sample(ps, u):
+    """envSampleWi for uniforms u (n, 4): (directions (n, 3), pdfs (n,), texels (n, 2) row, col)."""
+    uu = np.ascontiguousarray(u, np.float32).reshape(-1, 4)
+    n = uu.shape[0]
+    out = np.zeros((n, 4), np.float32)
+    tx = np.zeros((n, 2), np.int32)
+    s = scene_struct(ps)
+    prm = params()
+    assert lib().orc_env_sample(C.byref(s), C.byref(prm), _p(uu, C.c_float), n, _p(out, C.c_float),
+                                _p(tx, C.c_int32)) == 0
+    return out[:, :3], out[:, 3], tx
+
+
+def env_pdf(ps, wi):
+    w = np.ascontiguousarray(wi, np.float32).reshape(-1, 3)
+    out = np.zeros(w.shape[0], np.float32)
+    s = scene_struct(ps)
+    prm = params()
+    assert lib().orc_env_pdf(C.byref(s), C.byref(prm), _p(w, C.c_float), w.shape[0], _p(out, C.c_float)) == 0
+    return out
 
 
 def fmath(op, a, b=None):

@@ -532,6 +532,18 @@ static float halfToF(uint16_t h)
 
 struct Cnt { uint64_t bytes = 0, rows = 0, tests = 0, trav = 0; };
 
+// Estimator variants, for the tests that pin the restatement (0 = the reference's
+// estimators as written; set by orc_set_variant).
+//   kVarEmitCos    the triple tracer's camera pass weighs its strategies with the light's
+//                  actual emission pdf, cos/pi (lightSampleOneLe samples cosine-weighted
+//                  directions, light.glsl:111-120), instead of lightPdfLe's uniform
+//                  1/(2 pi) (light.glsl:101-109; triple_path_pass_pt.glsl:119, :162);
+//   kVarPlantPrev  a planted mistake: the s=1 weight drops its reverse-pdf factor
+//                  coefToPrev (triple_path_pass_pt.glsl:121);
+//   kVarPlantEnvSin a planted mistake: envSampleWi's pdf gets an extra sin(theta).
+constexpr int kVarEmitCos = 1, kVarPlantPrev = 2, kVarPlantEnvSin = 4;
+static int gVariant = 0;
+
 // Records of one sample of the differentiable naive estimator (replay with detached
 // sampling), kept by the oracle's FP64 adjoint.
 struct RepTap {
@@ -1215,10 +1227,18 @@ template <class R> struct Ctx {
     }
     V envLe(V wi) { return envAt(sphereToPlane(rotZ(wi, rotCN, rotSN))); }
     R envPortion(V wi) { return luminance(envLe(wi)) / R(s.env_sum); }
+    // the planted mistake kVarPlantEnvSin: a 1/sin(theta) Jacobian on top of the
+    // sin(theta)-weighted table (the reference has none, light.glsl:175-179)
+    R plantEnv(V wi, R pdf)
+    {
+        if (!(gVariant & kVarPlantEnvSin)) return pdf;
+        V l = rotZ(wi, rotCN, rotSN);
+        return pdf / std::sqrt(gmax(R(1e-12f), l.x * l.x + l.y * l.y));
+    }
     R envPdfLi(V wi)
     {
         if (s.env_sum == 0.0f) return R(0);
-        return envPortion(wi) * R(s.env_width) * R(s.env_height) * R(0.5f) * sq(M<R>::PiInv);
+        return plantEnv(wi, envPortion(wi) * R(s.env_width) * R(s.env_height) * R(0.5f) * sq(M<R>::PiInv));
     }
     R envProbAt(int x, int y)
     {
@@ -1230,7 +1250,9 @@ template <class R> struct Ctx {
         if (x < 0 || x > s.env_width || y < 0 || y >= s.env_height) return 0;
         return s.env_alias[(size_t)y * (s.env_width + 1) + x];
     }
-    Li envSampleLi(V x, R u0, R u1, R u2, R u3)
+    // envSampleWi (light.glsl:181-204): the row by the alias over the marginal kept in
+    // column w, then the column by the row's alias; the texel centre, rotated.
+    V envSampleWi(R u0, R u1, R u2, R u3, R& pdf, int* rowOut = nullptr, int* colOut = nullptr)
     {
         const int w = s.env_width, h = s.env_height;
         cn.bytes += 16;
@@ -1238,9 +1260,16 @@ template <class R> struct Ctx {
         int row = (u1 < envProbAt(w, rx)) ? rx : envAliasAt(w, rx);
         int cx = (int)(R(w) * u2);
         int col = (u3 < envProbAt(cx, row)) ? cx : envAliasAt(cx, row);
+        if (rowOut) { *rowOut = row; *colOut = col; }
         W uv((R(col) + R(0.5f)) / R(w), (R(row) + R(0.5f)) / R(h));
         V wi = rotZ(planeToSphere(uv), rotC, rotS);
-        R pdf = envPortion(wi) * R(w) * R(h) * R(0.5f) * sq(M<R>::PiInv);
+        pdf = plantEnv(wi, envPortion(wi) * R(w) * R(h) * R(0.5f) * sq(M<R>::PiInv));
+        return wi;
+    }
+    Li envSampleLi(V x, R u0, R u1, R u2, R u3)
+    {
+        R pdf;
+        V wi = envSampleWi(u0, u1, u2, u3, pdf);
         if (bvhTest(x + wi * R(1e-4f), wi, R(1e8f)) || pdf == R(0)) return Li{V(0), V(0), R(0)};
         return Li{wi, envLe(wi) / pdf, pdf};
     }
@@ -1685,9 +1714,12 @@ template <class R> struct Ctx {
                     R fp = evalPdf(t, mp, wo, wi, sf.ns);
                     R pdfRev = evalPdf(t, mp, wi, wo, sf.ns);
                     R pdfPLit = remap(pA);
-                    R cSurf = remap(R(0.5f) * M<R>::PiInv * absDot(sf.ns, wi));
+                    const R emitPdf = (gVariant & kVarEmitCos) ? satDot(nLit, -wi) * M<R>::PiInv
+                                                                : R(0.5f) * M<R>::PiInv;
+                    R cSurf = remap(emitPdf * absDot(sf.ns, wi));
                     R cLight = remap(fp * satDot(nLit, -wi));
-                    R cPrev = (bounce == 1) ? R(1) : remap(pdfRev * absDot(prevNorm, wo));
+                    R cPrev = (bounce == 1 || (gVariant & kVarPlantPrev)) ? R(1)
+                                                                        : remap(pdfRev * absDot(prevNorm, wo));
                     R cDist = remap(dist2);
                     R s1s0 = pdfPLit * cDist / cLight;
                     R t1s1x = t1s1 * cSurf * cPrev / cDist;
@@ -1713,7 +1745,8 @@ template <class R> struct Ctx {
                 // lightPdfLe (light.glsl:268-277)
                 V nl = surface(lid + s.obj_prim_count, np).ng;
                 R lpPos = R(1) / areaOf(lid + s.obj_prim_count);
-                R lpDir = (dot(nl, -bs.wi) <= R(0)) ? R(0) : R(0.5f) * M<R>::PiInv;
+                R lpDir = (dot(nl, -bs.wi) <= R(0)) ? R(0)
+                          : (gVariant & kVarEmitCos) ? dot(nl, -bs.wi) * M<R>::PiInv : R(0.5f) * M<R>::PiInv;
                 R pdfPLit = remap(lpPos * lightPdfPick(lid));
                 R cLight = remap(toNext * satDot(nLit, -bs.wi));
                 R cSurf = remap(lpDir * absDot(sf.ns, bs.wi));
@@ -2266,6 +2299,35 @@ void orc_sobol_matrices(uint32_t* out)
 }
 uint32_t orc_sobol_sample(uint32_t index, int32_t dim) { return sobol().sample(index, dim); }
 uint32_t orc_hash(uint32_t x) { return hashU(x); }
+void orc_hash_n(const uint32_t* in, uint32_t* out, int64_t n)
+{
+    for (int64_t i = 0; i < n; i++) out[i] = hashU(in[i]);
+}
+// envSampleWi for n 4-tuples of uniforms: direction and pdf (4 floats each) and the
+// sampled texel (row, column).
+int orc_env_sample(const orc_scene* s, const orc_params* p, const float* u, int64_t n, float* wi_pdf, int32_t* texel)
+{
+    orc::Cnt cn;
+    orc::Ctx<float> c(*s, *p, cn, s->sampler ? 1 : 0);
+    for (int64_t i = 0; i < n; i++) {
+        float pdf;
+        int row, col;
+        orc::V3<float> wi = c.envSampleWi(u[4 * i], u[4 * i + 1], u[4 * i + 2], u[4 * i + 3], pdf, &row, &col);
+        wi_pdf[4 * i] = wi.x; wi_pdf[4 * i + 1] = wi.y; wi_pdf[4 * i + 2] = wi.z; wi_pdf[4 * i + 3] = pdf;
+        texel[2 * i] = row; texel[2 * i + 1] = col;
+    }
+    return 0;
+}
+// envPdfLi (light.glsl:175-179) for n directions.
+int orc_env_pdf(const orc_scene* s, const orc_params* p, const float* wi, int64_t n, float* pdf)
+{
+    orc::Cnt cn;
+    orc::Ctx<float> c(*s, *p, cn, s->sampler ? 1 : 0);
+    for (int64_t i = 0; i < n; i++) pdf[i] = c.envPdfLi(orc::V3<float>(wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]));
+    return 0;
+}
+// Test-only estimator variants (kVar* above); 0 restores the reference's estimators.
+void orc_set_variant(int v) { gVariant = v; }
 
 // op: 0 sin, 1 cos, 2 atan2(in, in2), 3 log, 4 exp, 5 pow(in, in2)
 void orc_fmath(int op, const float* in, const float* in2, float* out, int64_t n)
