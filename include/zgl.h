@@ -226,16 +226,28 @@ zgl_status zgl_adjoint(zgl_context* ctx, const zgl_pass_params* params, const fl
  * naive integrator, recorded once as a HIP graph and replayed by zgl_batch_launch.  Each
  * frame receives exactly what zgl_render of the same params would write.  Waves take the
  * scenes in the configured order, one scene at a time per wave, pixels per wave from
- * that scene's counter.  The batch runs on its own stream: the contexts must not render
- * concurrently, and zgl_batch_synchronize must return before their frames are read. */
+ * that scene's counter.  The batch runs on its own stream, ordered after the work already
+ * enqueued on the contexts' streams when zgl_batch_launch is called (e.g. zgl_clear_frame)
+ * and before what they enqueue afterwards (zgl_read_frame waits for the batch).  The batch
+ * copies each context's scene, camera, noise and frame pointer at zgl_batch_set: after
+ * zgl_upload_scene, zgl_set_camera or zgl_set_noise on a context of the batch, or its
+ * zgl_destroy, zgl_batch_launch returns ZGL_E_STATE until zgl_batch_set is called again. */
 typedef struct zgl_batch zgl_batch;
 zgl_status zgl_batch_create(int32_t device, zgl_batch** out);
 void zgl_batch_destroy(zgl_batch* batch);
 const char* zgl_batch_last_error(const zgl_batch* batch);
 /* The scenes (contexts) and their passes (naive integrator); order = as given.  Allocates
- * the batch's sample values, sum of n_passes x width x height x 16 bytes; a batch above
- * 64 GiB of them is rejected (ZGL_E_INVALID). */
+ * the batch's sample values, sum of n_passes x width x height x 16 bytes, and a Sobol
+ * table of the union of the pass ranges (at most 65,537 passes of 1 KiB; points of passes
+ * past it are computed from the generator matrices).  Rejected (ZGL_E_INVALID): sample
+ * values above 64 GiB, and a scene whose work units, ceil(W/8) ceil(H/8) 64 n_passes,
+ * exceed 2^31 - 2^24. */
 zgl_status zgl_batch_set(zgl_batch* batch, zgl_context* const* contexts, const zgl_pass_params* params, int32_t n);
+/* The checks and sizes of zgl_batch_set without a device: film_wh holds n (width, height)
+ * pairs.  Returns the sample values' bytes and the Sobol table's first pass and pass count
+ * (any output may be NULL). */
+zgl_status zgl_batch_plan(const int32_t* film_wh, const zgl_pass_params* params, int32_t n, int64_t* sample_bytes,
+                          int32_t* sobol_first, int32_t* sobol_count);
 /* regen_threshold 1..63 (0 = default 20); blocks_per_cu 1..max (0 = max resident);
  * order: a permutation of 0..n-1 giving the launch order (NULL keeps the current one). */
 zgl_status zgl_batch_configure(zgl_batch* batch, int32_t regen_threshold, int32_t blocks_per_cu,

@@ -1,4 +1,5 @@
 """The C ABI library: symbols, error behaviour without a GPU, host packing semantics."""
+import ctypes as C
 import os
 import re
 
@@ -93,3 +94,36 @@ def test_camera_uniforms_orthonormal_and_roundtrip():
     for u, v in [(0.5, 0.5), (0.1, 0.9), (0.73, 0.21)]:
         uv = orc.camera_roundtrip(ps, u, v)
         assert np.allclose(uv, [u, v], atol=2e-6)
+
+
+def _plan(wh, passes, first=None):
+    n = len(wh)
+    films = np.ascontiguousarray(np.array(wh, np.int32).ravel())
+    params = (_lib.PassParams * n)()
+    for i in range(n):
+        params[i] = _lib.PassParams(_lib.INTEGRATOR_NAIVE, 0 if first is None else first[i], passes[i], 1, 8, 0, 1, 0,
+                                    0.5, 32, 1, 0, 0)
+    nb, sf, sc = C.c_int64(0), C.c_int32(0), C.c_int32(0)
+    st = _lib.lib().zgl_batch_plan(_lib.ptr(films, C.c_int32), params, n, C.byref(nb), C.byref(sf), C.byref(sc))
+    return st, nb.value, sf.value, sc.value
+
+
+def test_batch_plan_limits():
+    """The batch checks, without a device: each scene's (8x8 tile, pass) work units stay
+    below 2^31 - 2^24 (the persistent kernel counts them in int32), the sample values
+    below 64 GiB, and the Sobol table below 65,537 passes however far apart the scenes'
+    pass ranges are."""
+    st, nb, sf, sc = _plan([(1920, 1080), (128, 128)], [64, 4])
+    assert st == _lib.ZGL_OK and nb == (1920 * 1080 * 64 + 128 * 128 * 4) * 16 and (sf, sc) == (0, 65)
+    # 1920 x 1080 is 240 x 135 tiles of 64 units: 2,073,600 units per pass; 1000 passes fit
+    # the unit limit, 1036 do not (2^31 - 2^24 = 2,130,706,432)
+    assert _plan([(1920, 1080)], [1000])[0] == _lib.ZGL_OK
+    assert _plan([(1920, 1080)], [1036])[0] == _lib.ZGL_E_INVALID
+    # 64 GiB of sample values: 64 scenes x 1024^2 pixels x 64 passes x 16 B = 2^36 B exactly
+    assert _plan([(1024, 1024)] * 64, [64] * 64)[0] == _lib.ZGL_OK
+    assert _plan([(1024, 1024)] * 65, [1024] * 65)[0] == _lib.ZGL_E_INVALID
+    # far-apart pass ranges: the table is capped, the rest comes from the generator matrices
+    st, _, sf, sc = _plan([(64, 64), (64, 64)], [4, 4], first=[0, 1 << 24])
+    assert st == _lib.ZGL_OK and sf == 0 and sc == 65537
+    assert _plan([(0, 64)], [4])[0] == _lib.ZGL_E_INVALID
+    assert _plan([(64, 64)], [0])[0] == _lib.ZGL_E_INVALID

@@ -99,6 +99,7 @@ SIGNATURES = [
     ("zgl_batch_last_error", C.c_char_p, [C.c_void_p]),
     ("zgl_batch_set", C.c_int32, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(PassParams), C.c_int32]),
     ("zgl_batch_configure", C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, _ip]),
+    ("zgl_batch_plan", C.c_int32, [_ip, C.POINTER(PassParams), C.c_int32, C.POINTER(C.c_int64), _ip, _ip]),
     ("zgl_batch_launch", C.c_int32, [C.c_void_p]),
     ("zgl_batch_synchronize", C.c_int32, [C.c_void_p]),
     ("zgl_batch_tune", C.c_int32, [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),

@@ -1,10 +1,14 @@
 // zgl_capi.hip -- the C ABI: context, validated scene upload, pass dispatch.
 #include <algorithm>
+#include <atomic>
+#include <climits>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <string>
+#include <unordered_set>
 #include <vector>
 
 #include "../../include/zgl.h"
@@ -20,6 +24,13 @@ constexpr int kDefaultRegenThreshold = 20;
 constexpr size_t kAdjointLogBytes = (size_t)8 << 30;
 // Largest total of a batch's per-sample value buffers.
 constexpr size_t kBatchSampleBytes = (size_t)64 << 30;
+// Largest work-unit count of one scene of a batch ((8x8 tiles) x 64 x passes): the
+// persistent kernel counts units in int32, and each of its waves may take up to 64
+// indices past the end before it stops.
+constexpr int64_t kBatchMaxUnits = (int64_t)INT_MAX - (1 << 24);
+// Rows of a batch's precomputed Sobol table (1 KiB each); passes outside it compute
+// their points from the generator matrices.
+constexpr int kBatchSobolRows = 65537;
 
 namespace {
 
@@ -69,6 +80,14 @@ void hipCheck(hipError_t e, const char* what)
 
 void invalid(const std::string& m) { throw ZglError{ZGL_E_INVALID, m}; }
 
+// Generation stamps: every call that replaces a context's device buffers or the scene,
+// camera or noise a batch copies takes a new stamp, and a batch launch checks them.
+std::atomic<uint64_t> gGeneration{0};
+uint64_t nextGeneration() { return gGeneration.fetch_add(1) + 1; }
+// Contexts not yet destroyed (a batch launch refuses a destroyed context).
+std::mutex gLiveMutex;
+std::unordered_set<const void*> gLive;
+
 }  // namespace
 
 struct zgl_context {
@@ -96,6 +115,8 @@ struct zgl_context {
     DevScene scene = {};
     bool hasScene = false, hasCamera = false, hasNoise = false;
     int filmW = 0, filmH = 0;
+    uint64_t generation = 0;  // stamp of the last change a batch would have to see
+    void touch() { generation = nextGeneration(); }
 
     void upload(DevBuf& b, const void* src, size_t bytes, const char* what) { bufUpload(b, src, bytes, stream, what); }
     void ensure(DevBuf& b, size_t bytes, const char* what) { bufEnsure(b, bytes, what); }
@@ -116,7 +137,9 @@ struct zgl_batch {
     hipStream_t stream = nullptr;
     std::string err;
     std::vector<zgl_context*> ctxs;
+    std::vector<uint64_t> generations;  // per context: its stamp when the batch was set
     std::vector<zgl_pass_params> params;
+    hipEvent_t fence = nullptr;          // orders the batch after / before the contexts' streams
     std::vector<BatchScene> host;  // in launch order
     std::vector<int32_t> order;    // launch position -> index in ctxs
     DevBuf items, counters, sobolTab, res;
@@ -255,6 +278,9 @@ zgl_status zgl_create(int32_t device, zgl_context** out)
         hipCheck(hipMemsetAsync(ctx->counters.p, 0, NumCounters * sizeof(unsigned long long), ctx->stream),
                  "counters");
         hipCheck(hipStreamSynchronize(ctx->stream), "sync");
+        ctx->touch();
+        std::lock_guard<std::mutex> lock(gLiveMutex);
+        gLive.insert(ctx);
     });
     if (st != ZGL_OK) {
         ctx->releaseAll();
@@ -269,6 +295,10 @@ zgl_status zgl_create(int32_t device, zgl_context** out)
 void zgl_destroy(zgl_context* ctx)
 {
     if (!ctx) return;
+    {
+        std::lock_guard<std::mutex> lock(gLiveMutex);
+        gLive.erase(ctx);
+    }
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     ctx->releaseAll();
@@ -297,6 +327,7 @@ zgl_status zgl_synchronize(zgl_context* ctx)
 zgl_status zgl_upload_scene(zgl_context* ctx, const zgl_scene_buffers* sb)
 {
     return guarded(ctx, [&] {
+        ctx->touch();
         if (!sb) invalid("null scene");
         const int64_t nV = sb->n_vertices, nT = sb->n_triangles;
         if (nV < 3 || nT < 1 || !sb->vertices || !sb->normals || !sb->indices) invalid("empty geometry");
@@ -438,6 +469,7 @@ zgl_status zgl_upload_scene(zgl_context* ctx, const zgl_scene_buffers* sb)
 zgl_status zgl_set_camera(zgl_context* ctx, const zgl_camera* c)
 {
     return guarded(ctx, [&] {
+        ctx->touch();
         if (!c) invalid("null camera");
         if (c->film_width < 1 || c->film_height < 1 || c->film_width > 16384 || c->film_height > 16384)
             invalid("film size");
@@ -473,6 +505,7 @@ zgl_status zgl_set_camera(zgl_context* ctx, const zgl_camera* c)
 zgl_status zgl_set_noise(zgl_context* ctx, const float* noise, int32_t w, int32_t h)
 {
     return guarded(ctx, [&] {
+        ctx->touch();
         if (!noise || w < 1 || h < 1 || (int64_t)w * h > (1LL << 28)) invalid("noise texture");
         ctx->upload(ctx->noise, noise, (size_t)w * h * 8, "noise");
         hipCheck(hipStreamSynchronize(ctx->stream), "noise upload");
@@ -859,11 +892,13 @@ zgl_status zgl_batch_create(int32_t device, zgl_batch** out)
     b->device = device;
     zgl_status st = guarded(b, [&] {
         hipCheck(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
+        hipCheck(hipEventCreateWithFlags(&b->fence, hipEventDisableTiming), "hipEventCreate");
         hipCheck(hipDeviceGetAttribute(&b->numCUs, hipDeviceAttributeMultiprocessorCount, device), "attribute");
         b->maxBlocksPerCU = naiveBatchBlocksPerCU();
         b->blocksPerCU = b->maxBlocksPerCU;
     });
     if (st != ZGL_OK) {
+        if (b->fence) (void)hipEventDestroy(b->fence);
         if (b->stream) (void)hipStreamDestroy(b->stream);
         delete b;
         return st;
@@ -882,17 +917,55 @@ void zgl_batch_destroy(zgl_batch* b)
     b->counters.release();
     b->sobolTab.release();
     b->res.release();
+    if (b->fence) (void)hipEventDestroy(b->fence);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
 
 const char* zgl_batch_last_error(const zgl_batch* b) { return b ? b->err.c_str() : "null batch"; }
 
+// Checks a batch description: pass ranges, depth, each scene's work-unit count and the
+// total of sample values; returns the sample values' bytes and the Sobol table's range.
+struct BatchPlan {
+    size_t sampleBytes = 0;
+    int sobolFirst = 0, sobolCount = 0;
+};
+static BatchPlan batchPlan(const int32_t* filmWH, const zgl_pass_params* params, int n)
+{
+    if (!filmWH || !params || n < 1 || n > 65536) invalid("batch: contexts / params / count");
+    BatchPlan plan;
+    int first = INT_MAX;
+    int64_t end = 0;
+    size_t total = 0;
+    for (int i = 0; i < n; i++) {
+        const zgl_pass_params* p = &params[i];
+        const int w = filmWH[2 * i], h = filmWH[2 * i + 1];
+        if (w < 1 || h < 1 || w > 16384 || h > 16384) invalid("batch: film size");
+        if (p->integrator != ZGL_INTEGRATOR_NAIVE) invalid("batch: the batched launch runs the naive integrator");
+        if (p->n_passes < 1 || p->n_passes > 65536 || p->first_pass < 0 || p->first_pass > (1 << 24) ||
+            p->max_depth < 0 || p->max_depth > 1024)
+            invalid("batch: pass range / depth");
+        const int64_t units = (int64_t)((w + 7) / 8) * ((h + 7) / 8) * 64 * p->n_passes;
+        if (units > kBatchMaxUnits)
+            invalid("batch: a scene's (8x8 tile, pass) work units exceed 2^31 - 2^24; split its passes");
+        first = std::min(first, p->first_pass);
+        end = std::max(end, (int64_t)p->first_pass + p->n_passes);
+        total += (size_t)p->n_passes * (size_t)w * (size_t)h;
+    }
+    if (total * 16 > kBatchSampleBytes) invalid("batch: sample values exceed 64 GiB; split the batch");
+    plan.sampleBytes = total * 16;
+    plan.sobolFirst = first;
+    plan.sobolCount = (int)std::min<int64_t>(end - first + 1, kBatchSobolRows);
+    return plan;
+}
+
 // Builds the device table of the batch in launch order (b->order) and drops the graph.
 static void batchUpload(zgl_batch* b)
 {
     const int n = (int)b->ctxs.size();
     b->host.resize(n);
+    b->generations.resize(n);
+    for (int i = 0; i < n; i++) b->generations[i] = b->ctxs[i]->generation;
     for (int k = 0; k < n; k++) {
         const int i = b->order[k];
         zgl_context* ctx = b->ctxs[i];
@@ -927,30 +1000,27 @@ zgl_status zgl_batch_set(zgl_batch* b, zgl_context* const* ctxs, const zgl_pass_
 {
     return guarded(b, [&] {
         if (!ctxs || !params || n < 1 || n > 65536) invalid("batch: contexts / params / count");
-        int first = 1 << 30, end = 0;
+        std::vector<int32_t> wh(2 * (size_t)n);
         for (int i = 0; i < n; i++) {
             zgl_context* ctx = ctxs[i];
-            const zgl_pass_params* p = &params[i];
             if (!ctx) invalid("batch: null context");
             if (ctx->device != b->device) invalid("batch: context on another device");
             if (!ctx->hasScene || !ctx->hasCamera || !ctx->hasNoise)
                 throw ZglError{ZGL_E_STATE, "batch: scene, camera and noise must be set in every context"};
-            if (p->integrator != ZGL_INTEGRATOR_NAIVE) invalid("batch: the batched launch runs the naive integrator");
-            if (p->n_passes < 1 || p->n_passes > 65536 || p->first_pass < 0 || p->first_pass > (1 << 24) ||
-                p->max_depth < 0 || p->max_depth > 1024)
-                invalid("batch: pass range / depth");
             for (int j = 0; j < i; j++)
                 if (ctxs[j] == ctx) invalid("batch: a context appears twice");
-            first = std::min(first, p->first_pass);
-            end = std::max(end, p->first_pass + p->n_passes);
+            wh[2 * i] = ctx->filmW;
+            wh[2 * i + 1] = ctx->filmH;
         }
+        const BatchPlan plan = batchPlan(wh.data(), params, n);
         b->ctxs.assign(ctxs, ctxs + n);
         b->params.assign(params, params + n);
         b->order.resize(n);
         for (int i = 0; i < n; i++) b->order[i] = i;
-        // one Sobol table for the union of the pass ranges (plus one for dims past 256)
-        b->sobolFirst = first;
-        b->sobolCount = end - first + 1;
+        // one Sobol table for the union of the pass ranges (plus one for dims past 256),
+        // at most kBatchSobolRows rows
+        b->sobolFirst = plan.sobolFirst;
+        b->sobolCount = plan.sobolCount;
         b->ensure(b->sobolTab, (size_t)b->sobolCount * 256 * 4, "sobol table");
         hipCheck(launchSobolTable((const uint32_t*)ctxs[0]->sobolM.p, (uint32_t*)b->sobolTab.p, b->sobolFirst,
                                   b->sobolCount, b->stream),
@@ -967,7 +1037,6 @@ zgl_status zgl_batch_set(zgl_batch* b, zgl_context* const* ctxs, const zgl_pass_
             total += (size_t)params[i].n_passes * pixels;
             b->maxPixels = std::max(b->maxPixels, pixels);
         }
-        if (total * 16 > kBatchSampleBytes) invalid("batch: sample values exceed 64 GiB; split the batch");
         b->ensure(b->res, total * 16, "batch sample values");
         batchUpload(b);
     });
@@ -994,11 +1063,35 @@ zgl_status zgl_batch_configure(zgl_batch* b, int32_t regen_threshold, int32_t bl
     });
 }
 
+zgl_status zgl_batch_plan(const int32_t* film_wh, const zgl_pass_params* params, int32_t n, int64_t* sample_bytes,
+                          int32_t* sobol_first, int32_t* sobol_count)
+{
+    return guardedHost([&] {
+        const BatchPlan plan = batchPlan(film_wh, params, n);
+        if (sample_bytes) *sample_bytes = (int64_t)plan.sampleBytes;
+        if (sobol_first) *sobol_first = plan.sobolFirst;
+        if (sobol_count) *sobol_count = plan.sobolCount;
+    });
+}
+
 zgl_status zgl_batch_launch(zgl_batch* b)
 {
     return guarded(b, [&] {
         const int n = (int)b->ctxs.size();
         if (n == 0) throw ZglError{ZGL_E_STATE, "batch: no scenes set"};
+        {
+            std::lock_guard<std::mutex> lock(gLiveMutex);
+            for (int i = 0; i < n; i++)
+                if (!gLive.count(b->ctxs[i]) || b->ctxs[i]->generation != b->generations[i])
+                    throw ZglError{ZGL_E_STATE, "batch: a context was destroyed or its scene, camera or noise "
+                                                "changed since zgl_batch_set; call zgl_batch_set again"};
+        }
+        // after the work already enqueued on the contexts' streams (a frame clear, a
+        // restored frame) ...
+        for (zgl_context* ctx : b->ctxs) {
+            hipCheck(hipEventRecord(b->fence, ctx->stream), "batch order");
+            hipCheck(hipStreamWaitEvent(b->stream, b->fence, 0), "batch order");
+        }
         if (!b->exec) {
             // the whole launch (counter reset, persistent kernel, accumulation) as one graph
             hipCheck(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "capture");
@@ -1012,6 +1105,9 @@ zgl_status zgl_batch_launch(zgl_batch* b)
             hipCheck(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "graph instantiate");
         }
         hipCheck(hipGraphLaunch(b->exec, b->stream), "graph launch");
+        // ... and before what the contexts enqueue next (a frame read waits for the batch)
+        hipCheck(hipEventRecord(b->fence, b->stream), "batch order");
+        for (zgl_context* ctx : b->ctxs) hipCheck(hipStreamWaitEvent(ctx->stream, b->fence, 0), "batch order");
     });
 }
 

@@ -649,6 +649,39 @@ def config5(width=512, height=512, n=77, n_materials=32, env_size=(256, 128), se
     return d
 
 
+def config_materials(width=128, height=128, lens_radius=0.0, focal_dist=1.6):
+    """config1's Cornell box with one material of every BSDF branch: metal-workflow left
+    wall (roughness 0.4, metallic 0.5), rough dielectric short box (roughness 0.3, IOR
+    1.5), principled tall box with clearcoat 0.5, sheen and subsurface, a thin dielectric
+    pane (IOR 1.5) and a rough dielectric sphere (roughness 0.05, just above the delta
+    threshold); an optional thin-lens aperture."""
+    white = Material(base_color=WHITE)
+    floor = quad((0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1))
+    ceil = quad((0, 0, 1), (0, 1, 1), (1, 1, 1), (1, 0, 1), (0, 0, -1))
+    back = quad((0, 1, 0), (1, 1, 0), (1, 1, 1), (0, 1, 1), (0, -1, 0))
+    left = quad((0, 0, 0), (0, 1, 0), (0, 1, 1), (0, 0, 1), (1, 0, 0))
+    right = quad((1, 0, 0), (1, 0, 1), (1, 1, 1), (1, 1, 0), (-1, 0, 0))
+    short = box((0.66, 0.34, 0.0), (0.3, 0.3, 0.3), -17.0)
+    tall = box((0.34, 0.64, 0.0), (0.3, 0.3, 0.6), 17.0)
+    pane = quad((0.55, 0.15, 0.35), (0.9, 0.15, 0.35), (0.9, 0.2, 0.75), (0.55, 0.2, 0.75), (0, -0.995, 0.0995))
+    ball = uv_sphere((0.25, 0.3, 0.15), 0.12, 24, 13)
+    d = SceneDesc()
+    d.objects = [(merge([floor, ceil, back]), white),
+                 (left, Material(base_color=RED, roughness=0.4, metallic=0.5, type=MAT_METAL_WORKFLOW)),
+                 (right, Material(base_color=GREEN)),
+                 (short, Material(base_color=(0.95, 0.95, 1.0), roughness=0.3, ior=1.5, type=MAT_DIELECTRIC)),
+                 (tall, Material(base_color=(0.8, 0.6, 0.3), roughness=0.5, metallic=0.2, specular=0.6,
+                                 subsurface=0.3, sheen=0.5, clearcoat=0.5, clearcoat_gloss=0.5,
+                                 type=MAT_PRINCIPLED)),
+                 (pane, Material(base_color=(0.9, 1.0, 0.9), ior=1.5, type=MAT_THIN_DIELECTRIC)),
+                 (ball, Material(base_color=(1.0, 1.0, 1.0), roughness=0.05, ior=1.5, type=MAT_DIELECTRIC))]
+    d.lights = [(cornell_light(), (17.0, 12.0, 4.0))]
+    d.camera = CameraDesc(pos=(0.5, -1.5, 0.5), angle=(0.0, 0.0, 0.0), fov=40.0, lens_radius=lens_radius,
+                          focal_dist=focal_dist)
+    d.width, d.height = width, height
+    return d
+
+
 def cornell_tessellated(n, seed):
     """config1's Cornell box with every wall and box face an n x n grid (34 n^2 object
     triangles) and the boxes' rotations and positions jittered by the seed."""
@@ -669,15 +702,18 @@ def cornell_tessellated(n, seed):
     return objs
 
 
-def config4(n_scenes=64, seed=4, min_tris=1000, max_tris=2_000_000):
+def config4(n_scenes=64, seed=4, min_tris=1000, max_tris=2_000_000, only=None):
     """The batched workload: n_scenes Cornell boxes (config1 geometry tessellated to a
     triangle count log-uniform in [1k, 2M], boxes jittered per scene, ceiling light
     (17, 12, 4)), each with a resolution from {128^2, 256^2, 512^2, 1024^2}, spp from
-    U{4..64} and depth from U{3..10}.  Returns [(SceneDesc, spp, depth)]."""
+    U{4..64} and depth from U{3..10}.  Returns [(SceneDesc, spp, depth)]; `only` (a list
+    of scene numbers) returns just those scenes, drawn as in the whole workload."""
     rng = Rng(seed)
     out = []
     for k in range(n_scenes):
         u = rng.uniform(4)
+        if only is not None and k not in only:
+            continue
         tris = math.exp(math.log(min_tris) + u[0] * (math.log(max_tris) - math.log(min_tris)))
         n = max(1, int(round(math.sqrt(tris / 34.0))))
         res = (128, 256, 512, 1024)[min(3, int(u[1] * 4))]
