@@ -126,6 +126,19 @@ typedef struct zgl_camera {
 zgl_status zgl_set_camera(zgl_context* ctx, const zgl_camera* camera);
 /* Per-pixel seed texture (uNoiseTex, RG32F width x height, src/core/Sampler.cpp:66-80). */
 zgl_status zgl_set_noise(zgl_context* ctx, const float* noise_rg, int32_t width, int32_t height);
+/* Albedo textures (uTextures + uTexUVScale, src/core/Scene.cpp:256-258; Texture2DArray,
+ * src/core/Texture.cpp:134-171): n_layers sRGB8 RGB images of width x height texels (the
+ * array's size: every image's texels from (0, 0), the rest of its layer as the caller
+ * fills it; row 0 = t 0, i.e. the image's first row as stored), and per layer the
+ * (image size / array size) uv scale (2 floats).  A triangle whose matTexIndex holds
+ * texIndex k >= 0 in its upper 16 bits takes baseColor from layer k, read as
+ * texture2DArray(vec3(fract(uv) * scale_k, k)) at level 0: sRGB-decoded texels (FP64
+ * formula rounded to FP32), GL_LINEAR, GL_REPEAT over the array's size
+ * (material_loader.glsl:8-97).  n_layers = 0 removes the textures.  Rendering a scene that
+ * names a texture not set returns ZGL_E_STATE; the differentiable mode (zgl_render_replay,
+ * zgl_adjoint) rejects scenes with textured triangles (ZGL_E_INVALID). */
+zgl_status zgl_set_textures(zgl_context* ctx, const uint8_t* rgb8, int32_t n_layers, int32_t width, int32_t height,
+                            const float* uv_scale);
 
 /* Closest hit (bvhHit, src/shader/intersection.glsl:395-427) for rays in SoA layout
  * ox[n] oy[n] oz[n] dx[n] dy[n] dz[n].  prim = -1 and t = 1e8 on a miss; bary_bin =

@@ -37,6 +37,8 @@ class OrcScene(C.Structure):
         ("tan_fov", C.c_float), ("aspect", C.c_float), ("lens_radius", C.c_float), ("focal_dist", C.c_float),
         ("film_width", C.c_int32), ("film_height", C.c_int32),
         ("noise", _fp), ("noise_width", C.c_int32), ("noise_height", C.c_int32),
+        ("tex_rgb8", C.POINTER(C.c_uint8)), ("tex_layers", C.c_int32), ("tex_width", C.c_int32),
+        ("tex_height", C.c_int32), ("tex_scale", _fp),
     ]
 
 
@@ -78,6 +80,7 @@ def lib():
         L.orc_set_variant.restype = None
         L.orc_env_sample.argtypes = [C.POINTER(OrcScene), C.POINTER(OrcParams), _fp, C.c_int64, _fp, _ip]
         L.orc_env_pdf.argtypes = [C.POINTER(OrcScene), C.POINTER(OrcParams), _fp, C.c_int64, _fp]
+        L.orc_tex_lookup.argtypes = [C.POINTER(OrcScene), C.POINTER(OrcParams), C.c_int, _fp, C.c_int64, _fp]
         L.orc_fmath.argtypes = [C.c_int, _fp, _fp, _fp, C.c_int64]
         L.orc_fmath.restype = None
         L.orc_bsdf_eval_f64.argtypes = [_dp, C.c_int, _dp, _dp, _dp, _dp]
@@ -199,6 +202,17 @@ def env_pdf(ps, wi):
     return out
 
 
+def tex_lookup(ps, layer, uv):
+    """The albedo texture lookup of layer `layer` at uv (n, 2): (n, 3) linear RGB."""
+    u = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+    out = np.zeros((u.shape[0], 3), np.float32)
+    s = scene_struct(ps)
+    prm = params()
+    assert lib().orc_tex_lookup(C.byref(s), C.byref(prm), layer, _p(u, C.c_float), u.shape[0],
+                                _p(out, C.c_float)) == 0
+    return out
+
+
 def fmath(op, a, b=None):
     ops = {"sin": 0, "cos": 1, "atan2": 2, "log": 3, "exp": 4, "pow": 5}
     x = np.ascontiguousarray(a, np.float32)
@@ -247,6 +261,11 @@ def scene_struct(ps):
     s.film_width, s.film_height = cam.film_width, cam.film_height
     s.noise = _p(ps.noise, C.c_float)
     s.noise_width, s.noise_height = ps.noise.shape[1], ps.noise.shape[0]
+    tex = getattr(ps, "tex_rgb8", None)
+    if tex is not None:
+        s.tex_rgb8 = tex.ctypes.data_as(C.POINTER(C.c_uint8))
+        s.tex_layers, s.tex_height, s.tex_width = tex.shape[0], tex.shape[1], tex.shape[2]
+        s.tex_scale = _p(ps.tex_scale, C.c_float)
     return s
 
 

@@ -509,6 +509,8 @@ typedef struct orc_scene {
     float tan_fov, aspect, lens_radius, focal_dist;
     int32_t film_width, film_height;
     const float* noise; int32_t noise_width, noise_height;
+    // albedo textures: tex_layers sRGB8 RGB layers of tex_width x tex_height, uv scales
+    const uint8_t* tex_rgb8; int32_t tex_layers, tex_width, tex_height; const float* tex_scale;
 } orc_scene;
 
 typedef struct orc_params {
@@ -809,6 +811,33 @@ template <class R> struct Ctx {
         int i0 = wrap((int)fu, Wd), j0 = wrap((int)fv, H), i1 = wrap((int)fu + 1, Wd), j1 = wrap((int)fv + 1, H);
         R w00 = (R(1) - al) * (R(1) - be), w10 = al * (R(1) - be), w01 = (R(1) - al) * be, w11 = al * be;
         return ((envTex(i0, j0) * w00 + envTex(i1, j0) * w10) + envTex(i0, j1) * w01) + envTex(i1, j1) * w11;
+    }
+
+    // texture2DArray(uTextures, vec3(fract(uv) * uvScale, texId)) (material_loader.glsl:8-16):
+    // sRGB8 texels decoded by the GL_SRGB formula, GL_LINEAR at level 0, GL_REPEAT.
+    static float srgbDecode(uint8_t b)
+    {
+        const double c = b / 255.0;
+        return (float)(c <= 0.04045 ? c / 12.92 : std::pow((c + 0.055) / 1.055, 2.4));
+    }
+    V texel(int layer, int i, int j)
+    {
+        const uint8_t* t = s.tex_rgb8 + (((size_t)layer * s.tex_height + j) * s.tex_width + i) * 3;
+        return V(R(srgbDecode(t[0])), R(srgbDecode(t[1])), R(srgbDecode(t[2])));
+    }
+    V texAt(int layer, W uv)
+    {
+        cn.bytes += 8 + 16;
+        const int Wd = s.tex_width, H = s.tex_height;
+        const R su = (uv.x - std::floor(uv.x)) * R(s.tex_scale[2 * layer]);
+        const R sv = (uv.y - std::floor(uv.y)) * R(s.tex_scale[2 * layer + 1]);
+        R u = su * R(Wd) - R(0.5f), v = sv * R(H) - R(0.5f);
+        R fu = std::floor(u), fv = std::floor(v);
+        R al = u - fu, be = v - fv;
+        int i0 = wrap((int)fu, Wd), j0 = wrap((int)fv, H), i1 = wrap((int)fu + 1, Wd), j1 = wrap((int)fv + 1, H);
+        R w00 = (R(1) - al) * (R(1) - be), w10 = al * (R(1) - be), w01 = (R(1) - al) * be, w11 = al * be;
+        return ((texel(layer, i0, j0) * w00 + texel(layer, i1, j0) * w10) + texel(layer, i0, j1) * w01) +
+               texel(layer, i1, j1) * w11;
     }
 
     // --------------------------------------------------------------- BSDFs
@@ -1120,11 +1149,12 @@ template <class R> struct Ctx {
         std::memcpy(&t, s.materials + 16 * (size_t)m + 13, 4);
         return (uint32_t)t;
     }
-    P loadMat(uint32_t t, int m)
+    // material_loader.glsl:8-97; texId != -1 takes baseColor from the texture array
+    P loadMat(uint32_t t, int m, int texId = -1, W uv = W(0, 0))
     {
         const float* q = s.materials + 16 * (size_t)m;
         P p;
-        p.base = V(R(q[0]), R(q[1]), R(q[2]));
+        p.base = (texId == -1) ? V(R(q[0]), R(q[1]), R(q[2])) : texAt(texId, uv);
         if (t == 1) {
             cn.bytes += 48;
             p.rough = R(mixs(0.0134f, 1.0f, q[3]));
@@ -1381,7 +1411,7 @@ template <class R> struct Ctx {
             int mat = s.mat_tex_indices[id] & 0xffff;
             uint32_t t = matType(mat);
             if (t != 3 && t != 4 && dot(sf.ns, wo) < R(0)) { sf.ns = -sf.ns; sf.ng = -sf.ng; }
-            P mp = loadMat(t, mat);
+            P mp = loadMat(t, mat, s.mat_tex_indices[id] >> 16, sf.uv);
             if (p.sample_light) {
                 R ud = sample1D();
                 R u0 = sample1D(), u1 = sample1D(), u2 = sample1D(), u3 = sample1D();
@@ -1464,7 +1494,7 @@ template <class R> struct Ctx {
             int mat = s.mat_tex_indices[id] & 0xffff;
             uint32_t t = matType(mat);
             if (t != 3 && t != 4 && dot(sf.ns, wo) < R(0)) { sf.ns = -sf.ns; sf.ng = -sf.ng; }
-            P mp = loadMat(t, mat);
+            P mp = loadMat(t, mat, s.mat_tex_indices[id] >> 16, sf.uv);
             R ud = sample1D();
             R v0 = sample1D(), v1 = sample1D(), v2 = sample1D(), v3 = sample1D();
             if (p.sample_light) {
@@ -1558,9 +1588,9 @@ template <class R> struct Ctx {
             int mat = s.mat_tex_indices[id] & 0xffff;
             uint32_t t = matType(mat);
             if (t != 3 && t != 4 && dot(sf.ns, wo) < R(0)) { sf.ns = -sf.ns; sf.ng = -sf.ng; }
-            P mp = loadMat(t, mat);
+            P mp = loadMat(t, mat, s.mat_tex_indices[id] >> 16, sf.uv);
             const bool diff = differentiable(t);
-            P mpE = diff ? ce.loadMat(t, mat) : mp;
+            P mpE = diff ? ce.loadMat(t, mat, s.mat_tex_indices[id] >> 16, sf.uv) : mp;
             const int vi = rec ? (int)rec->vs.size() : 0;
             if (p.sample_light) {
                 R ud = sample1D();
@@ -1692,7 +1722,7 @@ template <class R> struct Ctx {
             int mat = s.mat_tex_indices[id] & 0xffff;
             uint32_t t = matType(mat);
             if (t != 3 && t != 4 && dot(sf.ns, wo) < R(0)) { sf.ns = -sf.ns; sf.ng = -sf.ng; }
-            P mp = loadMat(t, mat);
+            P mp = loadMat(t, mat, s.mat_tex_indices[id] >> 16, sf.uv);
             {
                 R l0 = sample1D(), l1 = sample1D();
                 int light = lightPick(W(l0, l1));
@@ -1839,7 +1869,7 @@ template <class R> struct Ctx {
             int mat = s.mat_tex_indices[id] & 0xffff;
             uint32_t t = matType(mat);
             if (t != 3 && t != 4 && dot(sf.ns, wo) < R(0)) { sf.ns = -sf.ns; sf.ng = -sf.ng; }
-            P mp = loadMat(t, mat);
+            P mp = loadMat(t, mat, s.mat_tex_indices[id] >> 16, sf.uv);
             {
                 R c0 = sample1D(), c1 = sample1D();
                 Ii ci = camSampleIi(pos, W(c0, c1));
@@ -1921,7 +1951,7 @@ template <class R> struct Ctx {
             R cPos = remap(prevPdfDir * absDot(sf.ns, wo));
             s0t1 = s0t1 / cPos;
             s1t1 = s1t1 / (cPos / (bounce == 1 ? remap(dist * dist) : R(1)));
-            P mp = loadMat(t, mat);
+            P mp = loadMat(t, mat, s.mat_tex_indices[id] >> 16, sf.uv);
             {
                 R c0 = sample1D(), c1 = sample1D();
                 Ii ci = camSampleIi(pos, W(c0, c1));
@@ -2324,6 +2354,18 @@ int orc_env_pdf(const orc_scene* s, const orc_params* p, const float* wi, int64_
     orc::Cnt cn;
     orc::Ctx<float> c(*s, *p, cn, s->sampler ? 1 : 0);
     for (int64_t i = 0; i < n; i++) pdf[i] = c.envPdfLi(orc::V3<float>(wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]));
+    return 0;
+}
+// texture2DArray(uTextures, vec3(fract(uv) * uvScale, layer)) for n uv pairs (3 floats each out).
+int orc_tex_lookup(const orc_scene* s, const orc_params* p, int layer, const float* uv, int64_t n, float* out)
+{
+    if (!s->tex_rgb8 || layer < 0 || layer >= s->tex_layers) return 1;
+    orc::Cnt cn;
+    orc::Ctx<float> c(*s, *p, cn, s->sampler ? 1 : 0);
+    for (int64_t i = 0; i < n; i++) {
+        orc::V3<float> v = c.texAt(layer, orc::V2<float>(uv[2 * i], uv[2 * i + 1]));
+        out[3 * i] = v.x; out[3 * i + 1] = v.y; out[3 * i + 2] = v.z;
+    }
     return 0;
 }
 // Test-only estimator variants (kVar* above); 0 restores the reference's estimators.

@@ -98,3 +98,57 @@ def test_triple(ctx, mats, rr):
     err = _rel_err(gpu, ref, floor)
     print(f"[gpu] triple: max rel err {err[sig].max():.3g}", flush=True)
     assert err[sig].max() <= 1e-4
+
+
+@pytest.fixture(scope="module")
+def textured():
+    return scene.pack(scene.config_textured(96, 96))
+
+
+@pytest.mark.parametrize("integrator", [_lib.INTEGRATOR_NAIVE, _lib.INTEGRATOR_GLOBAL_QUEUE,
+                                        _lib.INTEGRATOR_NAIVE_WAVEFRONT])
+def test_textured_camera_paths_bit_exact(ctx, textured, integrator):
+    """Albedo textures (texIndex != -1, material_loader.glsl:8-97): repeat past uv 1, a
+    layer smaller than the array, principled and metal-workflow materials."""
+    print(f"[gpu] textured scene, integrator {integrator}", flush=True)
+    ctx.upload(textured)
+    ctx.clear()
+    ctx.render(integrator, 1, 3, 2, 5, russian_roulette=True)
+    gpu = ctx.read_frame()[..., :3]
+    orc_int = {_lib.INTEGRATOR_GLOBAL_QUEUE: 3}.get(integrator, 0)
+    ref, _ = orc.render_pixels(textured, orc.params(orc_int, 1, 3, 2, 5, russian_roulette=True))
+    assert np.array_equal(gpu.view(np.uint32), ref.view(np.uint32))
+
+
+def test_textured_light_paths(ctx, textured):
+    print("[gpu] textured scene, light and triple", flush=True)
+    ctx.upload(textured)
+    ctx.clear()
+    ctx.render(_lib.INTEGRATOR_LIGHT, 0, 1, 1, 5, blocks_one_pass=16)
+    gpu = ctx.read_film()
+    ref, _ = orc.render_light(textured, orc.params(1, 0, 1, 1, 5, blocks_one_pass=16))
+    floor = 1e-3 * max(ref.mean(), 1e-12)
+    sig = ref > floor
+    assert _rel_err(gpu, ref, floor)[sig].max() <= 1e-4
+    ctx.clear()
+    ctx.render(_lib.INTEGRATOR_TRIPLE, 0, 2, 1, 5, blocks_one_pass=4)
+    gpu = ctx.read_film()
+    prm = orc.params(2, 0, 2, 1, 5, blocks_one_pass=4)
+    ref = orc.render_pixels(textured, prm)[0].astype(np.float64) + orc.render_light(textured, prm)[0]
+    floor = 1e-3 * max(ref.mean(), 1e-12)
+    sig = ref > floor
+    assert _rel_err(gpu, ref, floor)[sig].max() <= 1e-4
+
+
+def test_texture_state_errors(ctx, textured):
+    """A scene naming texture layers that are not set cannot render; the differentiable
+    mode refuses textured scenes."""
+    ctx.upload(textured)
+    ctx.set_textures(None, None)
+    with pytest.raises(_lib.ZglError) as e:
+        ctx.render(_lib.INTEGRATOR_NAIVE, 0, 1, 1, 3)
+    assert e.value.status == _lib.ZGL_E_STATE
+    ctx.upload(textured)
+    with pytest.raises(_lib.ZglError) as e:
+        ctx.render_replay(0, 1, 1, 3)
+    assert e.value.status == _lib.ZGL_E_INVALID

@@ -78,6 +78,7 @@ SIGNATURES = [
     ("zgl_upload_scene", C.c_int32, [C.c_void_p, C.POINTER(SceneBuffers)]),
     ("zgl_set_camera", C.c_int32, [C.c_void_p, C.POINTER(Camera)]),
     ("zgl_set_noise", C.c_int32, [C.c_void_p, _fp, C.c_int32, C.c_int32]),
+    ("zgl_set_textures", C.c_int32, [C.c_void_p, _u8p, C.c_int32, C.c_int32, C.c_int32, _fp]),
     ("zgl_trace_closest", C.c_int32, [C.c_void_p, _fp, C.c_int64, _ip, _fp, _up]),
     ("zgl_trace_closest_device", C.c_int32, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("zgl_trace_closest_counted", C.c_int32, [C.c_void_p, _fp, C.c_int64, _ip, _fp, _up]),

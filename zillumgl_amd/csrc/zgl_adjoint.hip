@@ -208,14 +208,15 @@ ZGL_D vec3 replayPath(const DevScene& s, const Replay& rp, const PassParams& prm
     vec3 wo = -ray.dir, result = v3(0.0f), throughput = v3(1.0f);
     for (int bounce = 1; bounce <= prm.maxDepth; bounce++) {
         SurfaceInfo surf = triangleSurfaceInfo<false>(s, id, pos, cn);
-        const int matId = s.matTexIndices[id] & 0x0000ffff;
+        const int matTex = s.matTexIndices[id];
+        const int matId = matTex & 0x0000ffff;
         const uint32_t matType = loadMaterialType<false>(s, matId, cn);
         if (matType != MatDielectric && matType != MatThinDielectric) {
             if (dot(surf.ns, wo) < 0.0f) flipNormals(surf);
         }
-        const BSDFParam mp = loadMaterial<false>(s, matType, matId, cn);
+        const BSDFParam mp = loadMaterial<false>(s, matType, matId, matTex >> 16, surf.uv, cn);
         const bool diff = differentiable(matType);
-        const BSDFParam mpE = diff ? loadMaterial<false>(se, matType, matId, cn) : mp;
+        const BSDFParam mpE = diff ? loadMaterial<false>(se, matType, matId, matTex >> 16, surf.uv, cn) : mp;
 
         if (prm.sampleLight) {
             const float ud = sample1D<false>(s, rng, cn);

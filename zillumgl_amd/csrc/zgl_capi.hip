@@ -104,6 +104,8 @@ struct zgl_context {
     DevBuf gqQueues, gqCounts;  // global-queue path tracer
     int gqBlocksPerCU = 0;
     DevBuf wfWork;    // wavefront schedule of the naive tracer
+    DevBuf tex, texScale, texRaw;  // texture array (linear RGB float4), per-layer uv scale, upload staging
+    int maxTexId = -1;             // largest texture id the uploaded scene's triangles use
     DevBuf naiveRes;  // per-sample values of the persistent naive tracer
     DevBuf hitLog;    // per-sample traversal results for the adjoint
     int wfTraceBlocksPerCU = 0, wfShadeBlocksPerCU = 0;
@@ -126,7 +128,7 @@ struct zgl_context {
                          &lightPower, &lightAlias, &lightProb, &env, &envAlias, &envProb, &noise, &sobolM,
                          &sobolTab, &frame, &film, &counters, &scratchRays, &scratchOut, &workCounter,
                          &evalMat, &evalEnv, &dFrame, &dMat, &dEnv, &bvhBounds, &gqQueues, &gqCounts, &wfWork,
-                         &naiveRes, &hitLog};
+                         &naiveRes, &hitLog, &tex, &texScale, &texRaw};
         for (DevBuf* b : all) b->release();
         bvh.release();
     }
@@ -346,8 +348,13 @@ zgl_status zgl_upload_scene(zgl_context* ctx, const zgl_scene_buffers* sb)
             invalid("environment map");
         for (int64_t i = 0; i < 3 * nT; i++)
             if (sb->indices[i] < 0 || sb->indices[i] >= nV) invalid("index out of range");
-        for (int64_t i = 0; i < sb->obj_prim_count; i++)
+        int maxTexId = -1;
+        for (int64_t i = 0; i < sb->obj_prim_count; i++) {
             if ((sb->mat_tex_indices[i] & 0xffff) >= sb->n_materials) invalid("material index out of range");
+            const int texId = sb->mat_tex_indices[i] >> 16;  // (texIndex << 16 | matIndex), texIndex -1: none
+            if (texId < -1) invalid("texture index out of range");
+            maxTexId = std::max(maxTexId, texId);
+        }
         const int64_t B = sb->bvh_size;
         // Finite bounds: the box test of general rays relies on NaN-free slab distances
         // (the device build checks the vertices instead).
@@ -417,6 +424,7 @@ zgl_status zgl_upload_scene(zgl_context* ctx, const zgl_scene_buffers* sb)
         ctx->upload(ctx->normals, sb->normals, (size_t)nV * 12, "normals");
         ctx->upload(ctx->texcoords, sb->texcoords, (size_t)sb->n_texcoords * 8, "texcoords");
         ctx->upload(ctx->matTex, sb->mat_tex_indices, (size_t)sb->obj_prim_count * 4, "matTexIndices");
+        ctx->maxTexId = maxTexId;
         ctx->upload(ctx->materials, sb->materials, (size_t)sb->n_materials * 64, "materials");
         ctx->upload(ctx->lightPower, sb->light_power, (size_t)nL * 12, "lightPower");
         ctx->upload(ctx->lightAlias, sb->light_alias, (size_t)nL * 4, "lightAlias");
@@ -516,6 +524,58 @@ zgl_status zgl_set_noise(zgl_context* ctx, const float* noise, int32_t w, int32_
     });
 }
 
+// The sRGB8 texture array of Scene::createGLContext (src/core/Scene.cpp:256-258;
+// Texture2DArray, src/core/Texture.cpp:134-171), decoded to linear FP32 on upload.
+zgl_status zgl_set_textures(zgl_context* ctx, const uint8_t* rgb8, int32_t n_layers, int32_t width, int32_t height,
+                            const float* uv_scale)
+{
+    return guarded(ctx, [&] {
+        ctx->touch();
+        if (n_layers == 0) {
+            ctx->scene.tex = nullptr;
+            ctx->scene.texScale = nullptr;
+            ctx->scene.nTextures = ctx->scene.texW = ctx->scene.texH = 0;
+            return;
+        }
+        if (!rgb8 || !uv_scale || n_layers < 0 || n_layers > 32767 || width < 1 || height < 1 || width > 16384 ||
+            height > 16384 || (int64_t)n_layers * width * height > (1LL << 28))
+            invalid("textures: layers / size");
+        for (int i = 0; i < 2 * n_layers; i++)
+            if (!(std::fabs(uv_scale[i]) <= 3.4028235e38f)) invalid("textures: non-finite uv scale");
+        const size_t n = (size_t)n_layers * width * height;
+        // the GL_SRGB decode of each 8-bit value, in FP64 rounded to FP32
+        float lut[256];
+        for (int b = 0; b < 256; b++) {
+            const double c = b / 255.0;
+            lut[b] = (float)(c <= 0.04045 ? c / 12.92 : std::pow((c + 0.055) / 1.055, 2.4));
+        }
+        const size_t rawBytes = (n * 3 + 255) & ~(size_t)255;
+        ctx->ensure(ctx->texRaw, rawBytes + sizeof(lut), "texture staging");
+        hipCheck(hipMemcpyAsync(ctx->texRaw.p, rgb8, n * 3, hipMemcpyHostToDevice, ctx->stream), "textures");
+        unsigned char* lutDev = static_cast<unsigned char*>(ctx->texRaw.p) + rawBytes;
+        hipCheck(hipMemcpyAsync(lutDev, lut, sizeof(lut), hipMemcpyHostToDevice, ctx->stream), "textures");
+        ctx->ensure(ctx->tex, n * 16, "textures");
+        hipCheck(launchSrgbExpand((const uint8_t*)ctx->texRaw.p, (const float*)lutDev, (float4*)ctx->tex.p, (int64_t)n,
+                                  ctx->stream),
+                 "textures");
+        ctx->upload(ctx->texScale, uv_scale, (size_t)n_layers * 8, "texture uv scale");
+        hipCheck(hipStreamSynchronize(ctx->stream), "textures");
+        ctx->scene.tex = (const float4*)ctx->tex.p;
+        ctx->scene.texScale = (const float2*)ctx->texScale.p;
+        ctx->scene.texW = width;
+        ctx->scene.texH = height;
+        ctx->scene.nTextures = n_layers;
+    });
+}
+
+// The scene's triangles name textures that have been set.
+static void checkTextures(const zgl_context* ctx)
+{
+    if (ctx->maxTexId >= ctx->scene.nTextures)
+        throw ZglError{ZGL_E_STATE, "the scene uses texture " + std::to_string(ctx->maxTexId) + " but " +
+                                        std::to_string(ctx->scene.nTextures) + " are set (zgl_set_textures)"};
+}
+
 // ---------------------------------------------------------------- tracing
 static void checkRays(zgl_context* ctx, int64_t n)
 {
@@ -588,8 +648,8 @@ zgl_status zgl_trace_any(zgl_context* ctx, const float* rays, const float* tmax,
 }
 
 // ---------------------------------------------------------------- rendering
-// Validates a pass request and builds the per-launch scene view and parameters.
-static void preparePasses(zgl_context* ctx, const zgl_pass_params* p, DevScene& s, PassParams& pp)
+// Validates a pass request.
+static void checkPasses(zgl_context* ctx, const zgl_pass_params* p)
 {
     if (!p) invalid("null params");
     if (!ctx->hasScene || !ctx->hasCamera || !ctx->hasNoise)
@@ -610,16 +670,13 @@ static void preparePasses(zgl_context* ctx, const zgl_pass_params* p, DevScene& 
         invalid("blocks_one_pass");
     if (p->integrator == ZGL_INTEGRATOR_TRIPLE && (p->loops_per_pass < 1 || p->loops_per_pass > 1024))
         invalid("loops_per_pass");
-    // Sobol points of the passes in this launch, plus one for dims spilling past 256.
-    const int first = p->first_pass, count = p->n_passes + 1;
-    ctx->ensure(ctx->sobolTab, (size_t)count * 256 * 4, "sobol table");
-    hipCheck(launchSobolTable((const uint32_t*)ctx->sobolM.p, (uint32_t*)ctx->sobolTab.p, first, count,
-                              ctx->stream),
-             "sobol table");
-    s = ctx->scene;
-    s.sobolTab = (const uint32_t*)ctx->sobolTab.p;
-    s.sobolTabFirst = first;
-    s.sobolTabCount = count;
+    checkTextures(ctx);
+}
+
+// The kernels' pass parameters of a request.
+static PassParams passParams(const zgl_pass_params* p)
+{
+    PassParams pp;
     pp.firstPass = p->first_pass;
     pp.nPasses = p->n_passes;
     pp.freeCounter = p->free_counter;
@@ -631,6 +688,24 @@ static void preparePasses(zgl_context* ctx, const zgl_pass_params* p, DevScene& 
     pp.loopsPerPass = p->loops_per_pass > 0 ? p->loops_per_pass : 1;
     pp.pathsPerPass = 1536 * (p->blocks_one_pass > 0 ? p->blocks_one_pass : 1);
     pp.lptScale = 0.0f;
+    return pp;
+}
+
+// Validates a pass request and builds the per-launch scene view and parameters.
+static void preparePasses(zgl_context* ctx, const zgl_pass_params* p, DevScene& s, PassParams& pp)
+{
+    checkPasses(ctx, p);
+    // Sobol points of the passes in this launch, plus one for dims spilling past 256.
+    const int first = p->first_pass, count = p->n_passes + 1;
+    ctx->ensure(ctx->sobolTab, (size_t)count * 256 * 4, "sobol table");
+    hipCheck(launchSobolTable((const uint32_t*)ctx->sobolM.p, (uint32_t*)ctx->sobolTab.p, first, count,
+                              ctx->stream),
+             "sobol table");
+    s = ctx->scene;
+    s.sobolTab = (const uint32_t*)ctx->sobolTab.p;
+    s.sobolTabFirst = first;
+    s.sobolTabCount = count;
+    pp = passParams(p);
 }
 
 zgl_status zgl_render(zgl_context* ctx, const zgl_pass_params* p)
@@ -735,6 +810,7 @@ zgl_status zgl_render_replay(zgl_context* ctx, const zgl_pass_params* p)
         PassParams pp;
         preparePasses(ctx, p, s, pp);
         if (p->integrator != ZGL_INTEGRATOR_NAIVE) invalid("the replay estimator is the naive integrator's");
+        if (ctx->maxTexId >= 0) invalid("the differentiable mode does not take textured materials");
         hipCheck(launchNaiveReplay(s, pp, (const float4*)ctx->evalMat.p, (const float4*)ctx->evalEnv.p,
                                    (float4*)ctx->frame.p, nullptr, nullptr, nullptr, 1, nullptr, nullptr, ctx->stream),
                  "replay pass");
@@ -745,6 +821,7 @@ zgl_status zgl_adjoint(zgl_context* ctx, const zgl_pass_params* p, const float* 
                        float* d_env_rgb)
 {
     return guarded(ctx, [&] {
+        if (ctx->maxTexId >= 0) invalid("the differentiable mode does not take textured materials");
         DevScene s;
         PassParams pp;
         preparePasses(ctx, p, s, pp);
@@ -1007,6 +1084,7 @@ zgl_status zgl_batch_set(zgl_batch* b, zgl_context* const* ctxs, const zgl_pass_
             if (ctx->device != b->device) invalid("batch: context on another device");
             if (!ctx->hasScene || !ctx->hasCamera || !ctx->hasNoise)
                 throw ZglError{ZGL_E_STATE, "batch: scene, camera and noise must be set in every context"};
+            checkTextures(ctx);
             for (int j = 0; j < i; j++)
                 if (ctxs[j] == ctx) invalid("batch: a context appears twice");
             wh[2 * i] = ctx->filmW;

@@ -52,6 +52,9 @@ struct DevScene {
     const int32_t* envAlias;     // (envW + 1) * envH
     const float* envProb;
     const float2* noise;         // filmW * filmH, RG32F
+    const float4* tex;           // texture array: nTextures x texH x texW texels, linear RGB
+                                 // (the sRGB8 layers decoded), row 0 = t 0
+    const float2* texScale;      // per layer: image size / array size (texUVScale)
     const uint32_t* sobolM;      // 256 * 32 generator columns
     const uint32_t* sobolTab;    // precomputed Sobol points for a pass range
     int64_t nVertices;
@@ -69,6 +72,7 @@ struct DevScene {
     int32_t sampler;               // 0: independent rand(), 1: Sobol
     int32_t sobolTabFirst, sobolTabCount;
     int32_t noiseW, noiseH;
+    int32_t texW, texH, nTextures;
     // camera (camera.glsl:3-12)
     vec3 camF, camR, camU, camPos;
     mat3 camMatInv;
@@ -430,6 +434,27 @@ template <bool C> ZGL_D vec3 envLookup(const DevScene& s, vec2 uv, Counters& cn)
     cnt<C>(cn, 24);
     float w00 = (1.0f - al) * (1.0f - be), w10 = al * (1.0f - be), w01 = (1.0f - al) * be, w11 = al * be;
     return ((t00 * w00 + t10 * w10) + t01 * w01) + t11 * w11;
+}
+
+// texture2DArray(uTextures, vec3(fract(uv) * texUVScale[texId], texId)) at level 0
+// (material_loader.glsl:8-16): GL_LINEAR magnification, GL_REPEAT over the array's size,
+// texels already decoded from sRGB (the GL_SRGB format decodes before filtering).
+template <bool C> ZGL_D vec3 textureLookup(const DevScene& s, int texId, vec2 uv, Counters& cn)
+{
+    const float2 sc = s.texScale[texId];
+    const int W = s.texW, H = s.texH;
+    const float4* layer = s.tex + (size_t)texId * W * H;
+    const float su = (uv.x - ffloorG(uv.x)) * sc.x, sv = (uv.y - ffloorG(uv.y)) * sc.y;
+    float u = su * (float)W - 0.5f, v = sv * (float)H - 0.5f;
+    float fu = ffloorG(u), fv = ffloorG(v);
+    float al = u - fu, be = v - fv;
+    int i0 = wrapRepeat((int)fu, W), j0 = wrapRepeat((int)fv, H);
+    int i1 = wrapRepeat((int)fu + 1, W), j1 = wrapRepeat((int)fv + 1, H);
+    const float4 a = layer[(size_t)j0 * W + i0], b = layer[(size_t)j0 * W + i1];
+    const float4 c = layer[(size_t)j1 * W + i0], d = layer[(size_t)j1 * W + i1];
+    cnt<C>(cn, 8 + 16);
+    float w00 = (1.0f - al) * (1.0f - be), w10 = al * (1.0f - be), w01 = (1.0f - al) * be, w11 = al * be;
+    return ((v3(a.x, a.y, a.z) * w00 + v3(b.x, b.y, b.z) * w10) + v3(c.x, c.y, c.z) * w01) + v3(d.x, d.y, d.z) * w11;
 }
 
 // ---------------------------------------------------------------------------
@@ -810,11 +835,13 @@ template <bool C> ZGL_D uint32_t loadMaterialType(const DevScene& s, int matId, 
     cnt<C>(cn, 16);
     return (uint32_t)__float_as_int(s.materials[matId * 4 + 3].y);
 }
-template <bool C> ZGL_D BSDFParam loadMaterial(const DevScene& s, uint32_t type, int matId, Counters& cn)
+// texId != -1 takes baseColor from the texture array (loadBaseColor, material_loader.glsl:8-16).
+template <bool C>
+ZGL_D BSDFParam loadMaterial(const DevScene& s, uint32_t type, int matId, int texId, vec2 uv, Counters& cn)
 {
     BSDFParam r = {};
     float4 t0 = s.materials[matId * 4 + 0];
-    r.baseColor = v3(t0.x, t0.y, t0.z);
+    r.baseColor = (texId == -1) ? v3(t0.x, t0.y, t0.z) : textureLookup<C>(s, texId, uv, cn);
     if (type == MatPrincipled) {
         float4 t1 = s.materials[matId * 4 + 1], t2 = s.materials[matId * 4 + 2];
         cnt<C>(cn, 48);

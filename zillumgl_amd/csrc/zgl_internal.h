@@ -39,6 +39,7 @@ struct BatchScene {
 
 hipError_t launchSobolTable(const uint32_t* sobolM, uint32_t* tab, int first, int count, hipStream_t st);
 hipError_t launchFill(float* p, int64_t n, float v, hipStream_t st);
+hipError_t launchSrgbExpand(const uint8_t* rgb, const float* lut, float4* out, int64_t n, hipStream_t st);
 hipError_t launchPostProcess(const float4* frame, float4* out, unsigned char* rgb8, int w, int h, float scale, int tone,
                              hipStream_t st);
 hipError_t launchFilmToFrame(const float* film, float4* frame, int w, int h, hipStream_t st);

@@ -62,6 +62,15 @@ __global__ void k_sobol_table(const uint32_t* __restrict__ M, uint32_t* __restri
     tab[g] = r;
 }
 
+// sRGB8 texels to linear FP32 through the 256-entry decode table.
+__global__ void k_srgb_expand(const uint8_t* __restrict__ rgb, const float* __restrict__ lut, float4* __restrict__ out,
+                              int64_t n)
+{
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    out[g] = make_float4(lut[rgb[3 * g]], lut[rgb[3 * g + 1]], lut[rgb[3 * g + 2]], 0.0f);
+}
+
 __global__ void k_fill(float* p, int64_t n, float v)
 {
     int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -194,13 +203,14 @@ __device__ void lightTrace(const DevScene& s, const PassParams& prm, Rng& rng, f
         if (id - s.objPrimCount >= 0) break;
         vec3 pos = rayPoint(ray, dist);
         SurfaceInfo surf = triangleSurfaceInfo<C>(s, id, pos, cn);
-        int matId = s.matTexIndices[id] & 0x0000ffff;
+        const int matTex = s.matTexIndices[id];
+        const int matId = matTex & 0x0000ffff;
         cnt<C>(cn, 4);
         uint32_t matType = loadMaterialType<C>(s, matId, cn);
         if (matType != MatDielectric && matType != MatThinDielectric) {
             if (dot(surf.ns, wo) < 0.0f) flipNormals(surf);
         }
-        BSDFParam mp = loadMaterial<C>(s, matType, matId, cn);
+        BSDFParam mp = loadMaterial<C>(s, matType, matId, matTex >> 16, surf.uv, cn);
         {
             CameraIiSample ci = cameraSampleIi(s, pos, sample2D<C>(s, rng, cn));
             if (ci.pdf > 0.0f) {
@@ -279,13 +289,14 @@ __device__ vec3 tripleCameraTrace(const DevScene& s, const PassParams& prm, Ray 
 
     for (int bounce = 1; bounce <= prm.maxDepth; bounce++) {
         if (bounce > 1) surf = triangleSurfaceInfo<C>(s, id, pos, cn);
-        int matId = s.matTexIndices[id] & 0x0000ffff;
+        const int matTex = s.matTexIndices[id];
+        const int matId = matTex & 0x0000ffff;
         cnt<C>(cn, 4);
         uint32_t matType = loadMaterialType<C>(s, matId, cn);
         if (matType != MatDielectric && matType != MatThinDielectric) {
             if (dot(surf.ns, wo) < 0.0f) flipNormals(surf);
         }
-        BSDFParam mp = loadMaterial<C>(s, matType, matId, cn);
+        BSDFParam mp = loadMaterial<C>(s, matType, matId, matTex >> 16, surf.uv, cn);
         {
             int light = lightSampleOne<C>(s, sample2D<C>(s, rng, cn), cn);
             int triId = light + s.objPrimCount;
@@ -425,7 +436,8 @@ __device__ void tripleLightTrace(const DevScene& s, const PassParams& prm, Rng& 
         if (id - s.objPrimCount >= 0) break;
         vec3 pos = rayPoint(ray, dist);
         SurfaceInfo surf = triangleSurfaceInfo<C>(s, id, pos, cn);
-        int matId = s.matTexIndices[id] & 0x0000ffff;
+        const int matTex = s.matTexIndices[id];
+        const int matId = matTex & 0x0000ffff;
         cnt<C>(cn, 4);
         uint32_t matType = loadMaterialType<C>(s, matId, cn);
         if (matType != MatDielectric && matType != MatThinDielectric) {
@@ -434,7 +446,7 @@ __device__ void tripleLightTrace(const DevScene& s, const PassParams& prm, Rng& 
         float coefToPos = remapP(prevPdfDir * absDot(surf.ns, wo));
         s0t1 /= coefToPos;
         s1t1 /= coefToPos / (bounce == 1 ? remapP(dist * dist) : 1.0f);
-        BSDFParam mp = loadMaterial<C>(s, matType, matId, cn);
+        BSDFParam mp = loadMaterial<C>(s, matType, matId, matTex >> 16, surf.uv, cn);
         {
             CameraIiSample ci = cameraSampleIi(s, pos, sample2D<C>(s, rng, cn));
             if (ci.pdf > 0.0f) {
@@ -509,6 +521,12 @@ hipError_t launchSobolTable(const uint32_t* M, uint32_t* tab, int first, int cou
 {
     if (count <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_sobol_table, dim3(gridFor((int64_t)count * 256, 256)), dim3(256), 0, st, M, tab, first, count);
+    return hipGetLastError();
+}
+hipError_t launchSrgbExpand(const uint8_t* rgb, const float* lut, float4* out, int64_t n, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_srgb_expand, dim3(gridFor(n, 256)), dim3(256), 0, st, rgb, lut, out, n);
     return hipGetLastError();
 }
 hipError_t launchFill(float* p, int64_t n, float v, hipStream_t st)

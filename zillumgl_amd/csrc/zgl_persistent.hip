@@ -206,13 +206,14 @@ __device__ __forceinline__ void naiveBody(const DevScene& s0, const PassParams& 
             if (phase == kSurface) {
                 const int id = wk.closest;
                 SurfaceInfo surf = triangleSurfaceInfo<C>(s, id, pos, cn);
-                const int matId = s.matTexIndices[id] & 0x0000ffff;
+                const int matTex = s.matTexIndices[id];
+                const int matId = matTex & 0x0000ffff;
                 cnt<C>(cn, 4);
                 const uint32_t matType = loadMaterialType<C>(s, matId, cn);
                 if (matType != MatDielectric && matType != MatThinDielectric) {
                     if (dot(surf.ns, wo) < 0.0f) flipNormals(surf);
                 }
-                const BSDFParam mp = loadMaterial<C>(s, matType, matId, cn);
+                const BSDFParam mp = loadMaterial<C>(s, matType, matId, matTex >> 16, surf.uv, cn);
                 bool shadow = false;
                 Ray shadowRay;
                 float shadowDist = 0.f;
@@ -589,12 +590,13 @@ __global__ __launch_bounds__(256, 4) void k_triple_persistent(DevScene s, PassPa
 
             if (phase == kSurface) {
                 SurfaceInfo surf = triangleSurfaceInfo<false>(s, id, pos, cn);
-                const int matId = s.matTexIndices[id] & 0x0000ffff;
+                const int matTex = s.matTexIndices[id];
+                const int matId = matTex & 0x0000ffff;
                 const uint32_t matType = loadMaterialType<false>(s, matId, cn);
                 if (matType != MatDielectric && matType != MatThinDielectric) {
                     if (dot(surf.ns, wo) < 0.0f) flipNormals(surf);
                 }
-                const BSDFParam mp = loadMaterial<false>(s, matType, matId, cn);
+                const BSDFParam mp = loadMaterial<false>(s, matType, matId, matTex >> 16, surf.uv, cn);
                 bool shadow = false;
                 Ray shadowRay;
                 float shadowDist = 0.f;

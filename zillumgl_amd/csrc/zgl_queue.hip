@@ -134,12 +134,13 @@ __device__ bool extendPath(const DevScene& s, const PassParams& prm, const Queue
     const float portion = prm.lightPortion;
 
     SurfaceInfo surf = triangleSurfaceInfo<false>(s, id, pos, cn);
-    const int matId = s.matTexIndices[id] & 0x0000ffff;
+    const int matTex = s.matTexIndices[id];
+    const int matId = matTex & 0x0000ffff;
     const uint32_t matType = loadMaterialType<false>(s, matId, cn);
     if (matType != MatDielectric && matType != MatThinDielectric) {
         if (dot(surf.ns, wo) < 0.0f) flipNormals(surf);
     }
-    const BSDFParam mp = loadMaterial<false>(s, matType, matId, cn);
+    const BSDFParam mp = loadMaterial<false>(s, matType, matId, matTex >> 16, surf.uv, cn);
 
     const float ud = sample1D<false>(s, rng, cn);
     const vec4 us = sample4D<false>(s, rng, cn);

@@ -375,12 +375,13 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_shade(DevScene s, PassParams pr
             flags = 0;
             if (alive && surface) {
                 SurfaceInfo surf = triangleSurfaceInfo<false>(s, id, pos, cn);
-                const int matId = s.matTexIndices[id] & 0x0000ffff;
+                const int matTex = s.matTexIndices[id];
+                const int matId = matTex & 0x0000ffff;
                 const uint32_t matType = loadMaterialType<false>(s, matId, cn);
                 if (matType != MatDielectric && matType != MatThinDielectric) {
                     if (dot(surf.ns, wo) < 0.0f) flipNormals(surf);
                 }
-                const BSDFParam mp = loadMaterial<false>(s, matType, matId, cn);
+                const BSDFParam mp = loadMaterial<false>(s, matType, matId, matTex >> 16, surf.uv, cn);
                 Ray shadowRay;
                 float shadowDist = 0.f;
                 bool pendValid = false;

@@ -46,7 +46,19 @@ class Context:
         self._check(_lib.lib().zgl_upload_scene(self.h, C.byref(sb)))
         self.set_camera(packed.camera)
         self.set_noise(packed.noise)
+        self.set_textures(getattr(packed, "tex_rgb8", None), getattr(packed, "tex_scale", None))
         return self
+
+    def set_textures(self, rgb8, uv_scale):
+        """Albedo texture array (layers, h, w, 3) uint8 sRGB and per-layer uv scales
+        (layers, 2); None removes the textures."""
+        if rgb8 is None:
+            self._check(_lib.lib().zgl_set_textures(self.h, None, 0, 0, 0, None))
+            return
+        t = np.ascontiguousarray(rgb8, np.uint8)
+        sc = np.ascontiguousarray(uv_scale, np.float32)
+        self._check(_lib.lib().zgl_set_textures(self.h, ptr(t, C.c_uint8), t.shape[0], t.shape[2], t.shape[1],
+                                                ptr(sc, C.c_float)))
 
     def set_camera(self, cam):
         self._check(_lib.lib().zgl_set_camera(self.h, C.byref(cam)))

@@ -55,6 +55,7 @@ class Mesh:
     normals: np.ndarray              # (n, 3) float32
     indices: np.ndarray              # (m, 3) uint32
     texcoords: np.ndarray = None     # (n, 2) float32 or None
+    texture: np.ndarray = None       # albedo image (h, w, 3) uint8, sRGB, row 0 first, or None
 
 
 @dataclass
@@ -273,8 +274,17 @@ def pack(desc: SceneDesc, bvh: str = "host") -> PackedScene:
     if bvh not in ("host", "device"):
         raise ValueError("bvh must be 'host' or 'device'")
     verts, norms, texs, inds, mti, mats = [], [], [], [], [], []
+    images, image_index = [], {}
     off_v, off_m, obj_prims = 0, 0, 0
     for mesh, mat in desc.objects:
+        # the mesh instance's albedo texture (Resource::addImage: one layer per image)
+        tex_index = -1
+        if mesh.texture is not None:
+            key = id(mesh.texture)
+            if key not in image_index:
+                image_index[key] = len(images)
+                images.append(np.ascontiguousarray(mesh.texture, np.uint8))
+            tex_index = image_index[key]
         mats.append(mat.record())
         n = mesh.positions.shape[0]
         verts.append(mesh.positions.astype(np.float32))
@@ -283,7 +293,7 @@ def pack(desc: SceneDesc, bvh: str = "host") -> PackedScene:
         texs.append(tc.astype(np.float32))
         inds.append(mesh.indices.astype(np.int64) + off_v)
         m = mesh.indices.shape[0]
-        mti.append(np.full(m, off_m + ((-1 << 16) | 0), np.int64))
+        mti.append(np.full(m, off_m + ((tex_index << 16) | 0), np.int64))
         off_v += n
         obj_prims += m
         off_m += 1
@@ -342,6 +352,20 @@ def pack(desc: SceneDesc, bvh: str = "host") -> PackedScene:
     ps.env_rotation = float(desc.env_rotation)
     ps.sampler = int(desc.sampler)
     ps.n_tex = n_tex
+    # Texture2DArray::Texture2DArray (src/core/Texture.cpp:134-171): layers of the largest
+    # width and height, each image from texel (0, 0) (the rest zero), uv scale = image size
+    # / array size in FP32
+    if images:
+        mw = max(im.shape[1] for im in images)
+        mh = max(im.shape[0] for im in images)
+        arr = np.zeros((len(images), mh, mw, 3), np.uint8)
+        scale = np.zeros((len(images), 2), np.float32)
+        for k, im in enumerate(images):
+            arr[k, :im.shape[0], :im.shape[1]] = im
+            scale[k] = (_f(im.shape[1]) / _f(mw), _f(im.shape[0]) / _f(mh))
+        ps.tex_rgb8, ps.tex_scale = np.ascontiguousarray(arr), scale
+    else:
+        ps.tex_rgb8, ps.tex_scale = None, None
     ps.camera = camera_uniforms(desc.camera, desc.width, desc.height)
     ps.width, ps.height = desc.width, desc.height
     ps.noise = np.ascontiguousarray(noise_texture(desc.width, desc.height, desc.noise_seed))
@@ -678,6 +702,47 @@ def config_materials(width=128, height=128, lens_radius=0.0, focal_dist=1.6):
     d.lights = [(cornell_light(), (17.0, 12.0, 4.0))]
     d.camera = CameraDesc(pos=(0.5, -1.5, 0.5), angle=(0.0, 0.0, 0.0), fov=40.0, lens_radius=lens_radius,
                           focal_dist=focal_dist)
+    d.width, d.height = width, height
+    return d
+
+
+def checker_texture(w, h, cells, c0, c1, seed=0):
+    """An sRGB8 (h, w, 3) checkerboard of `cells` x `cells` squares between colours c0 and
+    c1 (0..255), with per-texel jitter of a few levels."""
+    y, x = np.mgrid[0:h, 0:w]
+    k = ((x * cells // w) + (y * cells // h)) % 2
+    img = np.where(k[..., None] == 0, np.asarray(c0, np.float64), np.asarray(c1, np.float64))
+    jit = (uniform01(seed, w * h * 3).reshape(h, w, 3) - 0.5) * 12.0
+    return np.clip(np.round(img + jit), 0, 255).astype(np.uint8)
+
+
+def config_textured(width=96, height=96):
+    """config1's Cornell box with albedo textures: a 64 x 48 checkerboard on the back wall,
+    the same image on the floor with texture coordinates running to 2.5 (repeat) and a
+    32 x 32 one (a quarter of the texture array's area: uv scale 0.5 x 0.667) on the tall
+    box under a principled material and on the short box under a metal-workflow one."""
+    white = Material(base_color=WHITE)
+    t_big = checker_texture(64, 48, 8, (200, 40, 30), (30, 60, 220), seed=1)
+    t_small = checker_texture(32, 32, 4, (240, 230, 90), (20, 120, 40), seed=2)
+    floor = quad((0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1))
+    floor.texcoords = floor.texcoords * np.float32(2.5) - np.float32(0.25)
+    floor.texture = t_big
+    back = quad((0, 1, 0), (1, 1, 0), (1, 1, 1), (0, 1, 1), (0, -1, 0))
+    back.texture = t_big
+    ceil = quad((0, 0, 1), (0, 1, 1), (1, 1, 1), (1, 0, 1), (0, 0, -1))
+    left = quad((0, 0, 0), (0, 1, 0), (0, 1, 1), (0, 0, 1), (1, 0, 0))
+    right = quad((1, 0, 0), (1, 0, 1), (1, 1, 1), (1, 1, 0), (-1, 0, 0))
+    short = box((0.66, 0.34, 0.0), (0.3, 0.3, 0.3), -17.0)
+    short.texture = t_small
+    tall = box((0.34, 0.64, 0.0), (0.3, 0.3, 0.6), 17.0)
+    tall.texture = t_small
+    d = SceneDesc()
+    d.objects = [(floor, Material(base_color=WHITE, roughness=0.7, specular=0.5, type=MAT_PRINCIPLED)),
+                 (back, white), (ceil, white), (left, Material(base_color=RED)), (right, Material(base_color=GREEN)),
+                 (short, Material(roughness=0.5, metallic=0.4, type=MAT_METAL_WORKFLOW)),
+                 (tall, Material(roughness=0.4, metallic=0.0, specular=0.5, type=MAT_PRINCIPLED))]
+    d.lights = [(cornell_light(), (17.0, 12.0, 4.0))]
+    d.camera = CameraDesc(pos=(0.5, -1.5, 0.5), angle=(0.0, 0.0, 0.0), fov=40.0)
     d.width, d.height = width, height
     return d
 
