@@ -50,6 +50,17 @@ class Hip:
         assert self.h.hipDeviceSynchronize() == 0
 
 
+def profile_value(name, key):
+    """A `key: value` line of a counter summary under profiles/, or None."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith(key + ":"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def workload(name):
     from zillumgl_amd import scene
     if name == "c1":
@@ -65,7 +76,10 @@ def cpu_baseline(ps, depth, budget_s=20.0):
     """The CPU oracle (FP32 restatement) on the host cores: pass 0 (1 spp) over whole
     image rows, strided so the sample covers the full frame, until ~budget_s."""
     from oracle import oracle as orc
-    threads = min(16, os.cpu_count() or 1)
+    # every core this process may run on, up to the 16-core share a one-GPU machine
+    # gives each GPU (the rest belong to other GPUs' jobs)
+    nproc = len(os.sched_getaffinity(0))
+    threads = min(16, nproc)
     prm = orc.params(0, 0, 1, 1, depth)
     W, H = ps.width, ps.height
     t0 = time.perf_counter()
@@ -80,8 +94,9 @@ def cpu_baseline(ps, depth, budget_s=20.0):
             break
     dt = time.perf_counter() - t0
     return {"value": round(done / dt / 1e6, 5), "unit": "Msamples/s", "cores": threads, "kind": "port",
+            "nproc": nproc, "cpu_count": os.cpu_count(),
             "sample": f"{rows} strided full-width rows x 1 spp (pass 0) of the same frame = {done} samples, "
-                      f"{dt:.1f} s, FP32 oracle (oracle/zgl_oracle.cpp), {threads} threads"}
+                      f"{dt:.1f} s, FP32 oracle (oracle/zgl_oracle.cpp, g++ -O3), {threads} threads"}
 
 
 def adjoint_bench(hip, steps):
@@ -416,14 +431,20 @@ def main():
         algo_bytes = float(cnt[0])
         avg_s = (sum(kernel_ms) / len(kernel_ms)) / 1000.0
         achieved = algo_bytes / avg_s / 1e9
-        traffic = None
-        tf = os.path.join(ROOT, "profiles", f"naive_{args.workload}_traffic.md")
-        if os.path.exists(tf):
-            for line in open(tf):
-                if line.startswith("hbm_bytes_per_launch:"):
-                    traffic = float(line.split(":")[1])
+        traffic = profile_value(f"naive_{args.workload}_traffic.md", "hbm_bytes_per_launch")
+        lanes = profile_value(f"naive_{args.workload}_valu.md", "active_lanes_per_valu")
+        busy = profile_value(f"naive_{args.workload}_valu.md", "valu_issue_busy")
+        fabric = traffic / avg_s / 1e9 if traffic else None
+        # achieved/frac: the algorithmic bytes of the reference's data layout; L2 serves
+        # much of them, so the measured L2-to-HBM traffic is reported beside them, and
+        # the measured limiter (VALU issue, lane use) from the counter profile
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                    "achieved_is": "algorithmic bytes per launch / event-timed launch",
+                    "fabric_GBps": round(fabric, 1) if fabric else None,
+                    "fabric_frac": round(fabric / HBM_PEAK_GBS, 4) if fabric else None,
+                    "limiter": (f"VALU issue ({busy:.0%} busy, {lanes:.1f} of 64 lanes active per VALU instruction)"
+                                if lanes and busy else None),
                     "algorithmic_bytes_per_launch": algo_bytes,
                     "bytes_per_sample": round(algo_bytes / samples, 1),
                     "rows_per_traversal": round(float(cnt[1]) / max(float(cnt[3]), 1.0), 2),

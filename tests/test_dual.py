@@ -114,3 +114,53 @@ def test_dual_lambertian_jacobian_closed_form(probe):
     expect = np.zeros((3, 5), np.float32)
     expect[[0, 1, 2], [0, 1, 2]] = np.float32(1 / np.pi)
     assert np.allclose(jac, expect, rtol=1e-6, atol=0)
+
+
+def _reflect(d, m):
+    return 2.0 * (d @ m) * m - d
+
+
+def test_dual_ggx_tangents_near_the_specular_peak(probe):
+    """Near the specular peak of a smooth metal (raw roughness 0.02-0.12, half vector
+    within a few alpha of the normal) the FP32 GGX value keeps few digits (the half
+    vector's cos is rounded and c^2 (a^2 - 1) + 1 cancels): the FP32 value differs from
+    FP64 by up to ~10% there.  The roughness and baseColor tangents must still match FP64
+    central differences to 2e-4 of the larger of the derivative and its natural scale
+    (f / roughness, f / baseColor); tangents built on the FP32 value miss by ~1e-2."""
+    rng = np.random.default_rng(31)
+    worst = 0.0
+    checked = 0
+    for _ in range(400):
+        rec = np.zeros(16, np.float32)
+        rec[0:3] = rng.uniform(0.2, 1.0, 3)
+        rec[3] = rng.uniform(0.02, 0.12)
+        rec[5] = 1.0
+        rec[6] = 0.5
+        rec[12] = 1.5
+        rec[13] = np.array([1], np.int32).view(np.float32)[0]
+        n = rng.normal(size=3)
+        n /= np.linalg.norm(n)
+        wo = rng.normal(size=3)
+        wo -= min(0.0, wo @ n) * 2 * n
+        wo /= np.linalg.norm(wo)
+        r = 0.0134 * (1 - rec[3]) + rec[3]
+        a = r * r
+        m = n + rng.normal(size=3) * a * rng.uniform(0.2, 3.0)
+        m /= np.linalg.norm(m)
+        wi = _reflect(wo, m)
+        if wi @ n <= 0.05 or wo @ n <= 0.05:
+            continue
+        wo32, wi32, n32 = (x.astype(np.float32) for x in (wo, wi, n))
+        _, jac = probe(rec, 1, wo32, wi32, n32)
+        r64 = rec.astype(np.float64)
+        f64 = orc.bsdf_eval_f64(r64, 1, wo32, wi32, n32)
+        for k, slot, h in ((3, 3, 1e-7), (0, 0, 1e-6), (1, 1, 1e-6), (2, 2, 1e-6)):
+            rp, rm = r64.copy(), r64.copy()
+            rp[slot] += h
+            rm[slot] -= h
+            fd = (orc.bsdf_eval_f64(rp, 1, wo32, wi32, n32) - orc.bsdf_eval_f64(rm, 1, wo32, wi32, n32)) / (2 * h)
+            scale = np.abs(f64) / max(float(r64[slot]), 1e-3)
+            worst = max(worst, (np.abs(jac[:, k] - fd) / np.maximum(np.abs(fd), scale)).max())
+        checked += 1
+    assert checked > 200
+    assert worst < 2e-4, worst

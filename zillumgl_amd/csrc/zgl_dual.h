@@ -8,9 +8,11 @@
 // dual number carrying those five tangents.  Every function here repeats the float
 // evaluation of the corresponding BSDF (material.glsl:68-83 Lambertian, :357-555
 // principled BRDF) operation by operation, so the value part equals the float
-// evaluation bit for bit and the tangents are the exact derivatives of that float
-// program.  The host build is compared with the oracle's evaluation (values) and with
-// FP64 central differences (tangents).
+// evaluation bit for bit.  The tangents are the derivatives of the BSDF: those of the
+// float program, except in the specular lobe, where the float GGX term loses its digits
+// near a smooth lobe's peak and the tangents are formed in FP64 (dPrincipledMetal).  The
+// host build is compared with the oracle's evaluation (values) and with FP64 central
+// differences (tangents).
 #pragma once
 
 #include "zgl_math.h"
@@ -190,27 +192,59 @@ ZGL_HD D5 dSmithG(vec3 n, vec3 wo, vec3 wi, D5 alpha)
 {
     return dSchlickG(absDot(n, wo), alpha) * dSchlickG(absDot(n, wi), alpha);
 }
-ZGL_HD D5 dGgx(float cosTheta, D5 alpha)
+// ggx's float program (microfacet.glsl:4-12): the values of the differentiable BSDF.
+ZGL_HD float ggxValue(float cosTheta, float a2)
 {
-    if (cosTheta < 1e-6f) return dconst(0.0f);
-    D5 a2 = alpha * alpha;
-    D5 nom = a2;
-    D5 denom = cosTheta * cosTheta * (a2 - 1.0f) + 1.0f;
-    denom = denom * denom * Pi;
-    return nom / denom;
+    if (cosTheta < 1e-6f) return 0.0f;
+    float den = cosTheta * cosTheta * (a2 - 1.0f) + 1.0f;
+    den = den * den * Pi;
+    return a2 / den;
+}
+// GGX D and dD/d(a2) in FP64 at the FP32 directions, for the tangents:
+// D = a2 / (pi q^2), dD/da2 = (s2 - a2 c2) / (pi q^3), q = s2 + a2 c2, c = n.h, s2 = 1 - c^2.
+// Near a smooth lobe's peak (1 - c and a2 both ~1e-6) the float program keeps few digits
+// of q: the FP32 half vector's cos carries an absolute error of ~6e-8 and
+// c^2 (a2 - 1) + 1 cancels, so D is off by up to ~10% there; a tangent built on that
+// value is off by as much, while the FP64 form is exact to the inputs' precision.
+ZGL_HD void ggx64(vec3 wo, vec3 wi, vec3 n, float cosTheta, float a2f, double& D, double& dD)
+{
+    D = dD = 0.0;
+    if (cosTheta < 1e-6f) return;
+    const double hx = (double)wo.x + (double)wi.x, hy = (double)wo.y + (double)wi.y,
+                 hz = (double)wo.z + (double)wi.z;
+    const double c = ((double)n.x * hx + (double)n.y * hy + (double)n.z * hz) / sqrt(hx * hx + hy * hy + hz * hz);
+    const double a2 = a2f, c2 = c * c, s2 = 1.0 - c2, q = s2 + a2 * c2;
+    const double pi = 3.14159265358979323846;
+    D = a2 / (pi * q * q);
+    dD = (s2 - a2 * c2) / (pi * q * q * q);
 }
 
+// The specular lobe: values are the float program's bits; tangents are formed in FP64
+// from the FP64 D and dD/d(a2) of ggx64 and the float Fresnel and Smith factors.
 ZGL_HD DV3 dPrincipledMetal(vec3 wo, vec3 wi, vec3 n, const DV3& fm0, D5 alpha)
 {
     float cosWo = satDot(n, wo), cosWi = satDot(n, wi);
     vec3 h = normalize(wo + wi);
     if (cosWo < 1e-10f || cosWi < 1e-10f) return dv3c(v3(0.0f));
     DV3 fm = dSchlickF(absDot(h, wo), fm0);
-    D5 dm = dGgx(dot(n, h), alpha);
+    const float cosH = dot(n, h);
+    const D5 a2 = alpha * alpha;
+    const float dm = ggxValue(cosH, a2.v);
     D5 gm = dSmithG(n, wo, wi, alpha);
     float denom = 4.0f * cosWi * cosWo;
     if (denom < 1e-7f) return dv3c(v3(0.0f));
-    return fm * dm * gm / denom;
+    DV3 r = fm * dconst(dm) * gm / denom;
+    double D, dD;
+    ggx64(wo, wi, n, cosH, a2.v, D, dD);
+    const double G = gm.v;
+    D5* out[3] = {&r.x, &r.y, &r.z};
+    const D5* f[3] = {&fm.x, &fm.y, &fm.z};
+    for (int c = 0; c < 3; c++)
+        for (int k = 0; k < NumMatGrad; k++)
+            out[c]->d[k] = (float)(((double)f[c]->d[k] * D * G +
+                                    (double)f[c]->v * (dD * (double)a2.d[k] * G + D * (double)gm.d[k])) /
+                                   (double)denom);
+    return r;
 }
 ZGL_HD DV3 dPrincipledClearcoat(vec3 wo, vec3 wi, vec3 n, const DV3& baseColor, float alpha)
 {

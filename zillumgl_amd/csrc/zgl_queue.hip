@@ -94,7 +94,7 @@ __global__ __launch_bounds__(kQBlock) void k_gq_primary(DevScene s, int spp, uin
     int x, y;
     pixelOfTileIndex(blockIdx.x * blockDim.x + threadIdx.x, W, x, y);
     bool push = false;
-    QueueItem it;
+    QueueItem it = makeItem(v3(0.0f), v3(0.0f), v3(0.0f), 0, 0);
     if (x < W && y < H) {
         Counters cn{};
         const int pix = y * W + x;
@@ -188,13 +188,21 @@ __device__ bool extendPath(const DevScene& s, const PassParams& prm, const Queue
         accumulate(frame, pix, radiance * throughput * weight);
         return false;
     }
+    // Russian roulette as selects: the survival division is always evaluated (the same
+    // bits where it is kept) and the next item is always formed.  Written as a branch
+    // (`if (u >= p) return false; throughput /= p;`), ROCm 7.2's compiler stored a stale
+    // wo.z into the surviving paths' items (the register holding -wi.z was written only
+    // on the path without roulette), which the oracle comparisons caught as NaNs.
+    bool survive = true;
     if (prm.russianRoulette) {
         const float continueProb = fminG(maxComponent(bs.bsdf / bsdfPdf), 0.95f);
-        if (sample1D<false>(s, rng, cn) >= continueProb) return false;
-        throughput /= continueProb;
+        const float u = sample1D<false>(s, rng, cn);
+        survive = !(u >= continueProb);
+        const vec3 scaled = throughput / continueProb;
+        throughput = survive ? scaled : throughput;
     }
     out = makeItem(nextPos, throughput, -wi, pix, nextId);
-    return true;
+    return survive;
 }
 
 // Occupancy-sized grid; the block-uniform loop keeps whole waves together for the
@@ -208,7 +216,9 @@ __global__ __launch_bounds__(kQBlock) void k_gq_extend(DevScene s, PassParams pr
     for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
         const int i = base + threadIdx.x;
         bool push = false;
-        QueueItem out;
+        // initialised on every path: with `out` left unset where extendPath returns
+        // false, the compiler dropped the item's wo on the roulette's surviving path
+        QueueItem out = makeItem(v3(0.0f), v3(0.0f), v3(0.0f), 0, 0);
         if (i < n) push = extendPath(s, prm, qin[i], depth, spp, fc, frame, out);
         pushItem(qout, countOut, push, out);
     }
