@@ -183,6 +183,16 @@ typedef struct zgl_pass_params {
  * supported for it. */
 
 zgl_status zgl_render(zgl_context* ctx, const zgl_pass_params* params);
+/* zgl_render for ZGL_INTEGRATOR_NAIVE that allocates no device memory.  No counterpart in
+ * the reference: it is for callers that cannot allocate (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  The unit counters, Sobol points and
+ * per-sample values live in a static 128 MiB device array that each GPU holds until the
+ * process ends; passes run in chunks that fit it (at most 256 per chunk).  The frame is
+ * bit-identical to zgl_render's.  Rejects (ZGL_E_INVALID) a film whose one pass of
+ * sample values (W x H x 16 bytes) exceeds the 133,954,304 bytes left for them
+ * (8,372,144 pixels: 3840 x 2160 fits), and any other integrator.  Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (contexts on different streams) race; calls on one stream are ordered. */
+zgl_status zgl_render_static(zgl_context* ctx, const zgl_pass_params* params);
 /* Zero the frame (RGBA32F) and the light-splat film (planar R32F, 3W x H). */
 zgl_status zgl_clear_frame(zgl_context* ctx);
 /* Display transform of the frame (post_proc.glsl:1-59, called by Application::integrate,

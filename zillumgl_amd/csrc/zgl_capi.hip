@@ -32,6 +32,19 @@ constexpr int64_t kBatchMaxUnits = (int64_t)INT_MAX - (1 << 24);
 // their points from the generator matrices.
 constexpr int kBatchSobolRows = 65537;
 
+// Scratch of zgl_render_static, one array per GPU, held until the process ends:
+// [0, 256) the persistent schedule's unit counters, then the Sobol points of one chunk
+// of at most kStaticChunkPasses passes (+1 row for dims past 256), then the chunk's
+// per-sample values (16 B per pixel and pass).
+constexpr size_t kStaticScratchBytes = (size_t)128 << 20;
+constexpr int kStaticChunkPasses = 256;
+constexpr size_t kStaticSobolOffset = 256;
+constexpr size_t kStaticResOffset =
+    (kStaticSobolOffset + (size_t)(kStaticChunkPasses + 1) * 256 * 4 + 255) & ~(size_t)255;
+constexpr size_t kStaticResBytes = kStaticScratchBytes - kStaticResOffset;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
 namespace {
 
 struct DevBuf {
@@ -87,6 +100,27 @@ uint64_t nextGeneration() { return gGeneration.fetch_add(1) + 1; }
 // Contexts not yet destroyed (a batch launch refuses a destroyed context).
 std::mutex gLiveMutex;
 std::unordered_set<const void*> gLive;
+
+// Device address of `scratch` on each GPU (each GPU holds its own copy of the array),
+// resolved once per device by zgl_create with that device current.
+std::mutex gScratchMutex;
+unsigned char* gScratch[kMaxDevices] = {};
+
+void resolveScratch(int device)
+{
+    std::lock_guard<std::mutex> lock(gScratchMutex);
+    if (device < 0 || device >= kMaxDevices || gScratch[device]) return;
+    void* ptr = nullptr;
+    hipCheck(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "static scratch address");
+    if (!ptr) hipCheck(hipErrorInvalidSymbol, "static scratch address");
+    gScratch[device] = static_cast<unsigned char*>(ptr);
+}
+
+unsigned char* scratchOn(int device)
+{
+    std::lock_guard<std::mutex> lock(gScratchMutex);
+    return (device >= 0 && device < kMaxDevices) ? gScratch[device] : nullptr;
+}
 
 }  // namespace
 
@@ -273,6 +307,7 @@ zgl_status zgl_create(int32_t device, zgl_context** out)
     zgl_status st = guarded(ctx, [&] {
         hipCheck(hipStreamCreateWithFlags(&ctx->own, hipStreamNonBlocking), "hipStreamCreate");
         ctx->stream = ctx->own;
+        resolveScratch(device);
         ctx->upload(ctx->sobolM, sobolMatrices(), 256 * 32 * sizeof(uint32_t), "sobol matrices");
         ctx->ensure(ctx->counters, NumCounters * sizeof(unsigned long long), "counters");
         ctx->ensure(ctx->workCounter, 256, "work counter");
@@ -769,6 +804,49 @@ zgl_status zgl_render(zgl_context* ctx, const zgl_pass_params* p)
             sl.sampler = 0;  // TriplePath.cpp:72
             hipCheck(launchTripleLPT(sl, pp, film, cnt, ctx->stream), "triple light pass");
             hipCheck(launchFilmToFrame(film, frame, ctx->filmW, ctx->filmH, ctx->stream), "film copy");
+        }
+    });
+}
+
+// zgl_render for ZGL_INTEGRATOR_NAIVE with every buffer it writes besides the frame (unit
+// counters, Sobol points, per-sample values) in the GPU's static `scratch` array, so the
+// call allocates nothing. Passes run in chunks that fit the array; each chunk builds its
+// own Sobol rows, which hold the values the one-table form reads.
+zgl_status zgl_render_static(zgl_context* ctx, const zgl_pass_params* p)
+{
+    return guarded(ctx, [&] {
+        checkPasses(ctx, p);
+        if (p->integrator != ZGL_INTEGRATOR_NAIVE) invalid("zgl_render_static runs the naive integrator only");
+        unsigned char* base = scratchOn(ctx->device);
+        if (!base) throw ZglError{ZGL_E_STATE, "static scratch not resolved on this device"};
+        const size_t perPass = (size_t)ctx->filmW * ctx->filmH * 16;
+        if (perPass > kStaticResBytes)
+            invalid("zgl_render_static: a " + std::to_string(ctx->filmW) + "x" + std::to_string(ctx->filmH) +
+                    " film needs " + std::to_string(perPass) + " bytes of sample values per pass; the static scratch " +
+                    "holds " + std::to_string(kStaticResBytes));
+        const int chunk = (int)std::min<size_t>({(size_t)p->n_passes, kStaticResBytes / perPass,
+                                                 (size_t)kStaticChunkPasses});
+        if (!ctx->naiveBlocksPerCU) ctx->naiveBlocksPerCU = naivePersistentBlocksPerCU();
+        const int threshold = p->regen_threshold > 0 ? p->regen_threshold : kDefaultRegenThreshold;
+        int* counter = reinterpret_cast<int*>(base);
+        uint32_t* sobol = reinterpret_cast<uint32_t*>(base + kStaticSobolOffset);
+        float4* res = reinterpret_cast<float4*>(base + kStaticResOffset);
+        unsigned long long* cnt = p->count_bytes ? (unsigned long long*)ctx->counters.p : nullptr;
+        const PassParams pp = passParams(p);
+        DevScene s = ctx->scene;
+        s.sobolTab = sobol;
+        for (int k0 = 0; k0 < pp.nPasses; k0 += chunk) {
+            PassParams q = pp;
+            q.firstPass = pp.firstPass + k0;
+            q.freeCounter = pp.freeCounter + k0;
+            q.nPasses = std::min(chunk, pp.nPasses - k0);
+            s.sobolTabFirst = q.firstPass;
+            s.sobolTabCount = q.nPasses + 1;
+            hipCheck(launchSobolTable((const uint32_t*)ctx->sobolM.p, sobol, q.firstPass, q.nPasses + 1, ctx->stream),
+                     "sobol table");
+            hipCheck(launchNaivePersistent(s, q, (float4*)ctx->frame.p, res, q.nPasses, counter, threshold,
+                                           ctx->numCUs * ctx->naiveBlocksPerCU, cnt, ctx->stream),
+                     "naive pass");
         }
     });
 }
