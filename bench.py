@@ -275,6 +275,15 @@ def integrators_bench(hip, reps):
         key = "light_Mpaths_s" if integ == _lib.INTEGRATOR_LIGHT else f"{name}_Msamples_s"
         out[key] = round(unit_count * passes / t / 1e3, 2)
         out[f"{name}_ms"] = round(t, 3)
+        if integ != _lib.INTEGRATOR_GLOBAL_QUEUE:
+            # algorithmic bytes of the call (reference layout) from the counting variants
+            ctx.clear()
+            ctx.reset_counters()
+            ctx.render(integ, 0, passes, 1, depth, blocks_one_pass=blocks, loops_per_pass=1, count_bytes=True)
+            ctx.synchronize()
+            nbytes = float(ctx.counters()[0])
+            out[f"{name}_bytes_per_unit"] = round(nbytes / (unit_count * passes), 1)
+            out[f"{name}_algorithmic_GBps"] = round(nbytes / (t / 1e3) / 1e9, 1)
     ctx.close()
     return out
 
@@ -342,7 +351,8 @@ def rmse_vs_converged(ctx, W, H, spp, depth, ref_spp):
             "mean_radiance": round(float(ref.mean()), 6),
             "relative_rmse": round(mse_corrected ** 0.5 / float(ref.mean()), 6),
             "note": "same estimator and Sobol indices on the CPU oracle (frames equal per pixel to 1e-4), so "
-                    "the CPU baseline's RMSE is this one"}
+                    "the CPU baseline's RMSE is this one; the FP32 converged image is unbiased against the FP64 "
+                    "estimator at the same samples (tests/test_gpu_rmse.py)"}
 
 
 def main():
@@ -358,14 +368,22 @@ def main():
     ap.add_argument("--no-bvh", action="store_true")
     ap.add_argument("--no-integrators", action="store_true")
     ap.add_argument("--bvh-only", action="store_true", help="only the MTBVH build line (profiling)")
-    ap.add_argument("--reference-spp", type=int, default=0, metavar="N",
-                    help="also report the RMSE against an N-spp converged image (N a multiple of 64)")
+    ap.add_argument("--reference-spp", type=int, default=4096, metavar="N",
+                    help="RMSE against an N-spp converged image (N a multiple of 64; 0 = skip)")
     ap.add_argument("--threshold", type=int, default=0, help="persistent schedule regeneration threshold (0 = default)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES (PMC calibration) and exit")
+    ap.add_argument("--only", choices=["adjoint", "batch", "integrators"],
+                    help="only run that secondary line and print it (profiling)")
     args = ap.parse_args()
     if args.bvh_only:
         print(json.dumps(bvh_bench(max(args.steps, 5))))
+        return
+    if args.only:
+        line = {"adjoint": lambda: adjoint_bench(Hip(), max(args.steps, 3)),
+                "batch": lambda: batch_bench(max(args.steps, 3)),
+                "integrators": lambda: integrators_bench(Hip(), max(args.steps, 3))}[args.only]()
+        print(json.dumps(line))
         return
     if args.copy_probe:
         import torch

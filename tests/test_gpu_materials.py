@@ -152,3 +152,23 @@ def test_texture_state_errors(ctx, textured):
     with pytest.raises(_lib.ZglError) as e:
         ctx.render_replay(0, 1, 1, 3)
     assert e.value.status == _lib.ZGL_E_INVALID
+
+
+@pytest.mark.parametrize("threshold", [1, 20, 63])
+def test_light_persistent_equals_one_thread_per_path(ctx, mats, threshold):
+    """The light tracer's persistent schedule (regenerating lanes, walks advanced a few
+    rows per round, connections splatted after their visibility walk) and its
+    one-thread-per-path form (the byte-counting variant) splat the same contributions:
+    films equal within 1e-4 (FP32 atomic order), with roulette, at several regeneration
+    thresholds."""
+    print(f"[gpu] materials light persistent vs one thread per path, threshold {threshold}", flush=True)
+    ctx.upload(mats)
+    films = []
+    for count in (False, True):
+        ctx.clear()
+        ctx.render(_lib.INTEGRATOR_LIGHT, 2, 3, 4, 6, russian_roulette=True, blocks_one_pass=16,
+                   count_bytes=count, regen_threshold=threshold)
+        films.append(ctx.read_film().astype(np.float64))
+    floor = 1e-3 * max(films[1].mean(), 1e-12)
+    sig = films[1] > floor
+    assert sig.sum() > 1000 and _rel_err(films[0], films[1], floor)[sig].max() <= 1e-4

@@ -124,16 +124,29 @@ struct Replay {
 // One sample of the naive estimator with detached sampling (see the file comment).
 // ADJ = false returns the contribution; ADJ = true replays it with `rest` = that
 // contribution and accumulates delta-weighted gradients.
-// FP64 running sum of a sample's terms.
+// Running sum of a sample's terms, exact: an FP64 sum (x) plus the rounding errors of its
+// additions (ex), kept with Knuth's TwoSum.  The replay subtracts the terms as it meets
+// them, and the remainder -- the later terms, which a small factor may have made 1e-16 of
+// the earlier ones -- must survive that cancellation.
 struct Sum3 {
-    double x, y, z;
+    double x, y, z, ex, ey, ez;
 };
+ZGL_D void twoSumAdd(double& s, double& e, double b)
+{
+    const double a = s;
+    s = a + b;
+    const double bp = s - a;
+    e += (a - (s - bp)) + (b - bp);
+}
 ZGL_D void acc(Sum3& a, vec3 t, double sign)
 {
-    a.x += sign * (double)t.x;
-    a.y += sign * (double)t.y;
-    a.z += sign * (double)t.z;
+    twoSumAdd(a.x, a.ex, sign * (double)t.x);
+    twoSumAdd(a.y, a.ey, sign * (double)t.y);
+    twoSumAdd(a.z, a.ez, sign * (double)t.z);
 }
+ZGL_D double remX(const Sum3& a) { return a.x + a.ex; }
+ZGL_D double remY(const Sum3& a) { return a.y + a.ey; }
+ZGL_D double remZ(const Sum3& a) { return a.z + a.ez; }
 
 // Traversal results of one sample.  With `pre` (the forward kernel's log of the sample,
 // written by launchNaiveLogged), both passes read the results from it.  Without, the
@@ -181,10 +194,18 @@ template <bool ADJ> ZGL_D bool loggedTest(const DevScene& s, const Ray& r, float
 
 template <bool ADJ>
 ZGL_D vec3 replayPath(const DevScene& s, const Replay& rp, const PassParams& prm, vec2 scr, Rng& rng, Sum3& rest,
-                      vec3 delta, const Grads& gr, HitLog& log)
+                      Sum3& restSkip, vec3 delta, const Grads& gr, HitLog& log)
 {
-    // first pass: rest accumulates the terms; replay: rest loses them as they are met
+    // first pass: rest accumulates the terms; replay: rest loses them as they are met.
+    // restSkip does the same with the terms formed from tpSkip, the throughput whose
+    // first vanishing factor per channel is taken as 1.  A BSDF channel vanishes at 0 (a
+    // baseColor channel at 0 and no other lobe) or below 1e-20 of the vertex's largest
+    // channel (a metal's zero channel lit only by the Fresnel term (1 - c)^5 at wi ~ wo):
+    // rest / f cannot recover the terms after such a factor (they are 0 or underflow in
+    // FP32), and the derivative through that factor, which does not vanish, needs them.
     const double sign = ADJ ? -1.0 : 1.0;
+    vec3 tpSkip = v3(1.0f);
+    int zeroSeen = 0;  // bit c: channel c has met its first vanishing factor
     Counters cn;
     const bool uniform = prm.lightEnvUniform != 0;
     const float portion = prm.lightPortion;
@@ -267,6 +288,7 @@ ZGL_D vec3 replayPath(const DevScene& s, const Replay& rp, const PassParams& prm
                     const vec3 term = f * throughput * satDot(surf.ns, ls.wi) * ls.coef * weight;
                     result += term;
                     acc(rest, term, sign);
+                    acc(restSkip, f * tpSkip * satDot(surf.ns, ls.wi) * ls.coef * weight, sign);
                     if (ADJ) {
                         if (diff) {
                             const DV3 fd = evalDual(matType, rp.evalMat, matId, wo, ls.wi, surf.ns);
@@ -284,15 +306,30 @@ ZGL_D vec3 replayPath(const DevScene& s, const Replay& rp, const PassParams& prm
         const bool delta_ = (bs.flag == FlagSpecRefl || bs.flag == FlagSpecTrans);
         if (bs.pdf < 1e-8f) break;
         const vec3 fE = diff ? materialBSDF(matType, mpE, wo, bs.wi, surf.ns, ModeRadiance) : bs.bsdf;
+        // vanishing channels of this vertex's factor (see restSkip)
+        const float fMax = fmaxG(fabsG(fE.x), fmaxG(fabsG(fE.y), fabsG(fE.z)));
+        const bool vx = !(fabsG(fE.x) > 1e-20f * fMax), vy = !(fabsG(fE.y) > 1e-20f * fMax),
+                   vz = !(fabsG(fE.z) > 1e-20f * fMax);
         if (ADJ && diff) {
             // every later term carries the factor fE_c; rest holds their exact sum
+            // where fE_c vanishes (and no earlier factor of channel c did), the later terms
+            // with this factor taken as 1 are what restSkip still holds
             const DV3 fd = evalDual(matType, rp.evalMat, matId, wo, bs.wi, surf.ns);
-            const vec3 ratio = v3(fE.x != 0.0f ? (float)(rest.x / (double)fE.x) : 0.0f,
-                                  fE.y != 0.0f ? (float)(rest.y / (double)fE.y) : 0.0f,
-                                  fE.z != 0.0f ? (float)(rest.z / (double)fE.z) : 0.0f);
+            const vec3 ratio =
+                v3(vx ? ((zeroSeen & 1) ? 0.0f : (float)remX(restSkip)) : (float)(remX(rest) / (double)fE.x),
+                   vy ? ((zeroSeen & 2) ? 0.0f : (float)remY(restSkip)) : (float)(remY(rest) / (double)fE.y),
+                   vz ? ((zeroSeen & 4) ? 0.0f : (float)remZ(restSkip)) : (float)(remZ(rest) / (double)fE.z));
             addMatGrad(gr, matId, delta * ratio, fd);
         }
-        throughput *= fE / bs.pdf * (delta_ ? 1.0f : absDot(surf.ns, bs.wi));
+        const float geo = delta_ ? 1.0f : absDot(surf.ns, bs.wi);
+        {
+            const vec3 factor = fE / bs.pdf * geo;
+            const float one = 1.0f / bs.pdf * geo;  // the factor with fE_c taken as 1
+            const bool zx = vx && !(zeroSeen & 1), zy = vy && !(zeroSeen & 2), zz = vz && !(zeroSeen & 4);
+            tpSkip = tpSkip * v3(zx ? one : factor.x, zy ? one : factor.y, zz ? one : factor.z);
+            zeroSeen |= (zx ? 1 : 0) | (zy ? 2 : 0) | (zz ? 4 : 0);
+        }
+        throughput *= fE / bs.pdf * geo;
         const float rrProb = fminG(maxComponent(bs.bsdf / bs.pdf), 0.95f);
 
         float dist;
@@ -311,6 +348,7 @@ ZGL_D vec3 replayPath(const DevScene& s, const Replay& rp, const PassParams& prm
             const vec3 term = radiance * throughput * weight;
             result += term;
             acc(rest, term, sign);
+            acc(restSkip, radiance * tpSkip * weight, sign);
             if (ADJ) addEnvGrad(gr, tap, delta * throughput * weight);
             break;
         } else if (lightId >= 0) {
@@ -324,15 +362,22 @@ ZGL_D vec3 replayPath(const DevScene& s, const Replay& rp, const PassParams& prm
             const vec3 term = radiance * throughput * weight;
             result += term;
             acc(rest, term, sign);
+            acc(restSkip, radiance * tpSkip * weight, sign);
             break;
         }
-        if (prm.russianRoulette) {
-            if (sample1D<false>(s, rng, cn) >= rrProb) break;
-            throughput /= rrProb;
-        }
+        // the next vertex's state is set before the roulette, and the roulette is written
+        // as selects: a surviving path's state assigned after a roulette branch has been
+        // compiled wrongly (the global-queue tracer's extension)
         id = nextId;
         pos = nextPos;
         wo = -bs.wi;
+        if (prm.russianRoulette) {
+            const bool survive = !(sample1D<false>(s, rng, cn) >= rrProb);
+            const vec3 tpS = throughput / rrProb, skS = tpSkip / rrProb;
+            throughput = survive ? tpS : throughput;
+            tpSkip = survive ? skS : tpSkip;
+            if (!survive) break;
+        }
     }
     return result;
 }
@@ -383,8 +428,8 @@ __global__ __launch_bounds__(256) void k_naive_replay(DevScene s, PassParams prm
             rng.sampleOffset = sampleOffsetOf(spp);
             rng.dim = 0;
             const Rng start = rng;
-            Sum3 rest = {0.0, 0.0, 0.0};
-            const vec3 c = replayPath<false>(s, rp, prm, scr, rng, rest, v3(0.0f), gr, log);
+            Sum3 rest = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, restSkip = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+            const vec3 c = replayPath<false>(s, rp, prm, scr, rng, rest, restSkip, v3(0.0f), gr, log);
             if (hasNan(c)) continue;
             if (MODE == 0) {
                 const vec3 last = (spp == 0) ? v3(0.0f) : v3(mem.x, mem.y, mem.z);
@@ -393,7 +438,7 @@ __global__ __launch_bounds__(256) void k_naive_replay(DevScene s, PassParams prm
                 stored = true;
             } else if (delta.x != 0.0f || delta.y != 0.0f || delta.z != 0.0f) {
                 rng = start;
-                replayPath<true>(s, rp, prm, scr, rng, rest, delta, gr, log);
+                replayPath<true>(s, rp, prm, scr, rng, rest, restSkip, delta, gr, log);
             }
         }
         if (MODE == 0 && stored) frame[pix] = mem;

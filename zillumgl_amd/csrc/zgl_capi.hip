@@ -148,6 +148,7 @@ struct zgl_context {
     int numCUs = 0;
     int naiveBlocksPerCU = 0;
     int tripleBlocksPerCU = 0;
+    int lightBlocksPerCU = 0, tripleLightBlocksPerCU = 0;
     DevScene scene = {};
     bool hasScene = false, hasCamera = false, hasNoise = false;
     int filmW = 0, filmH = 0;
@@ -780,7 +781,16 @@ zgl_status zgl_render(zgl_context* ctx, const zgl_pass_params* p)
                      "global-queue pass");
         } else if (p->integrator == ZGL_INTEGRATOR_LIGHT) {
             s.sampler = 0;  // LightPath.cpp:49 forces the independent sampler
-            hipCheck(launchLight(s, pp, film, cnt, ctx->stream), "light pass");
+            if (cnt) {
+                // the byte-counting variant: one thread per light path
+                hipCheck(launchLight(s, pp, film, cnt, ctx->stream), "light pass");
+            } else {
+                if (!ctx->lightBlocksPerCU) ctx->lightBlocksPerCU = lightPersistentBlocksPerCU(false);
+                const int threshold = p->regen_threshold > 0 ? p->regen_threshold : kDefaultRegenThreshold;
+                hipCheck(launchLightPersistent(s, pp, false, film, (int*)ctx->workCounter.p, threshold,
+                                               ctx->numCUs * ctx->lightBlocksPerCU, ctx->stream),
+                         "light pass");
+            }
             hipCheck(launchFilmToFrame(film, frame, ctx->filmW, ctx->filmH, ctx->stream), "film copy");
         } else {
             // uScale = W*H / (blocks * loops * 1536)  (TriplePath.cpp:76-77)
@@ -802,7 +812,15 @@ zgl_status zgl_render(zgl_context* ctx, const zgl_pass_params* p)
             }
             DevScene sl = s;
             sl.sampler = 0;  // TriplePath.cpp:72
-            hipCheck(launchTripleLPT(sl, pp, film, cnt, ctx->stream), "triple light pass");
+            if (cnt) {
+                hipCheck(launchTripleLPT(sl, pp, film, cnt, ctx->stream), "triple light pass");
+            } else {
+                if (!ctx->tripleLightBlocksPerCU) ctx->tripleLightBlocksPerCU = lightPersistentBlocksPerCU(true);
+                const int threshold = p->regen_threshold > 0 ? p->regen_threshold : kDefaultRegenThreshold;
+                hipCheck(launchLightPersistent(sl, pp, true, film, (int*)ctx->workCounter.p, threshold,
+                                               ctx->numCUs * ctx->tripleLightBlocksPerCU, ctx->stream),
+                         "triple light pass");
+            }
             hipCheck(launchFilmToFrame(film, frame, ctx->filmW, ctx->filmH, ctx->stream), "film copy");
         }
     });

@@ -80,6 +80,22 @@ struct DevScene {
     int32_t filmW, filmH;
 };
 
+// The scene of a persistent kernel whose first argument is its DevScene `arg`, read
+// through the kernel-argument segment (where that first argument lies at offset 0) with
+// an index the compiler cannot fold: each field is then a scalar load where it is used,
+// instead of ~90 SGPRs of scene fields held across the kernel's whole loop, which
+// spilled SGPRs into VGPR lanes and pushed 14 VGPRs of the naive tracer to scratch.
+ZGL_D const DevScene& sceneArg(const DevScene& arg)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    int i = 0;
+    asm volatile("" : "+s"(i));
+    return ((const DevScene*)__builtin_amdgcn_kernarg_segment_ptr())[i];
+#else
+    return arg;
+#endif
+}
+
 // Algorithmic-byte and visit counters, kept in registers and flushed once per thread.
 struct Counters {
     uint32_t bytes = 0, rows = 0, tests = 0, traversals = 0;
@@ -1137,6 +1153,18 @@ ZGL_D CameraIiSample cameraSampleIi(const DevScene& s, vec3 ref, vec2 u)
     float lensArea = cameraDelta(s) ? 1.0f : Pi * s.lensRadius * s.lensRadius;
     float pdf = dist * dist / (cosTheta * lensArea);
     return CameraIiSample{wi, Ie, dist, uv, pdf};
+}
+// Film splat of the light-path integrators (light_path_integ.glsl:34-43, addImage); the
+// film is planar R32F of (3W) x H, added to with FP32 atomics.
+ZGL_D void splatFilm(const DevScene& s, float* film, vec2 uv, vec3 res)
+{
+    if (!inFilmBound(uv)) return;
+    int ix = (int)(uv.x * (float)s.filmW), iy = (int)(uv.y * (float)s.filmH);
+    if (ix >= s.filmW || iy >= s.filmH) return;  // out-of-range image atomics are dropped
+    float* p = film + (size_t)iy * 3 * s.filmW + 3 * ix;
+    unsafeAtomicAdd(p + 0, res.x);
+    unsafeAtomicAdd(p + 1, res.y);
+    unsafeAtomicAdd(p + 2, res.z);
 }
 struct CameraPdf { float pdfPos, pdfDir; };
 ZGL_D CameraPdf cameraPdfIe(const DevScene& s, const Ray& ray)

@@ -94,6 +94,12 @@ void wavefrontBlocks(int& traceBlocksPerCU, int& shadeBlocksPerCU);
 hipError_t launchNaiveWavefront(const DevScene& s, const PassParams& p, float4* frame, void* work, int batch,
                                 int traceBlocks, int shadeBlocks, hipStream_t st);
 hipError_t launchEnvExpand(const ushort4* env, float4* out, int64_t n, hipStream_t st);
+// The light tracer (triple = false) or the triple tracer's light pass (triple = true)
+// on the persistent schedule (zgl_lightpath.hip): p.nPasses passes of p.pathsPerPass
+// units, splatted into the planar film; counter: one int of work queue.
+hipError_t launchLightPersistent(const DevScene& s, const PassParams& p, bool triple, float* film, int* counter,
+                                 int threshold, int blocks, hipStream_t st);
+int lightPersistentBlocksPerCU(bool triple);
 hipError_t launchLight(const DevScene& s, const PassParams& p, float* film, unsigned long long* counters,
                        hipStream_t st);
 hipError_t launchTriplePT(const DevScene& s, const PassParams& p, float* film, unsigned long long* counters,

@@ -155,20 +155,6 @@ __global__ __launch_bounds__(kBlock) void k_trace_any(DevScene s, const float* _
     occ[i] = bvhTest<false>(s, r, tmax[i], cn) ? 1 : 0;
 }
 
-// ---------------------------------------------------------------------------
-// Film splats (light_path_integ.glsl:34-43); the film is planar R32F of (3W) x H.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void splat(const DevScene& s, float* film, vec2 uv, vec3 res)
-{
-    if (!inFilmBound(uv)) return;
-    int ix = (int)(uv.x * (float)s.filmW), iy = (int)(uv.y * (float)s.filmH);
-    if (ix >= s.filmW || iy >= s.filmH) return;  // out-of-range image atomics are dropped
-    float* p = film + (size_t)iy * 3 * s.filmW + 3 * ix;
-    unsafeAtomicAdd(p + 0, res.x);
-    unsafeAtomicAdd(p + 1, res.y);
-    unsafeAtomicAdd(p + 2, res.z);
-}
-
 // light_path_integ.glsl:45-146
 template <bool C>
 __device__ void lightTrace(const DevScene& s, const PassParams& prm, Rng& rng, float* film, Counters& cn)
@@ -187,7 +173,7 @@ __device__ void lightTrace(const DevScene& s, const PassParams& prm, Rng& rng, f
             if (visible<C>(s, pLit, pCam, cn)) {
                 vec3 Le = lightLe<C>(s, light, pLit, ci.wi, cn);
                 vec3 contrib = Le * ci.Ii / (ci.pdf * pdfPos * pdfSource);
-                if (!isBlack(contrib)) { splat(s, film, ci.uv, contrib); cnt<C>(cn, 12); }
+                if (!isBlack(contrib)) { splatFilm(s, film, ci.uv, contrib); cnt<C>(cn, 12); }
             }
         }
         LightLeSample le = lightSampleOneLe<C>(s, light, sample4D<C>(s, rng, cn), cn);
@@ -220,7 +206,7 @@ __device__ void lightTrace(const DevScene& s, const PassParams& prm, Rng& rng, f
                     float cosWi = satDot(surf.ng, ci.wi) * fabsG(dot(surf.ns, wo) / dot(surf.ng, wo));
                     vec3 res = ci.Ii * f * throughput * cosWi / ci.pdf;
                     if (!hasNan(res) && !fisnan(ci.pdf) && ci.pdf > 1e-8f && !isBlack(res)) {
-                        splat(s, film, ci.uv, res);
+                        splatFilm(s, film, ci.uv, res);
                         cnt<C>(cn, 12);
                     }
                 }
@@ -463,7 +449,7 @@ __device__ void tripleLightTrace(const DevScene& s, const PassParams& prm, Rng& 
                     float weight = 1.0f / (s0t1 * coef0 + s1t1 * coef1 + 1.0f);
                     vec3 res = contrib * weight;
                     if (!hasNan(res) && !fisnan(ci.pdf) && ci.pdf > 1e-8f && !isBlack(res)) {
-                        splat(s, film, ci.uv, res * prm.lptScale);
+                        splatFilm(s, film, ci.uv, res * prm.lptScale);
                         cnt<C>(cn, 12);
                     }
                 }

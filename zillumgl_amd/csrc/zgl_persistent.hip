@@ -106,7 +106,7 @@ __device__ __forceinline__ void naiveBody(const DevScene& s0, const PassParams& 
 
     for (;;) {
         const int csu = BATCH ? __builtin_amdgcn_readfirstlane(cs) : 0;
-        const DevScene& s = BATCH ? items[csu].s : s0;
+        const DevScene& s = BATCH ? items[csu].s : sceneArg(s0);
         const PassParams& prm = BATCH ? items[csu].prm : prm0;
         float4* const res = BATCH ? items[csu].res : res0;
         int* const pixelCounter = BATCH ? sceneCounters + csu : counter0;
@@ -485,14 +485,14 @@ __global__ __launch_bounds__(256, 4) void k_naive_batch(const BatchScene* __rest
 // walk and added if the walk finds no occluder.  Each sample's value goes to
 // res[pass * W * H + pixel]; k_triple_accumulate adds them to the planar film.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256, 4) void k_triple_persistent(DevScene s, PassParams prm, float4* __restrict__ res,
+__global__ __launch_bounds__(256, 4) void k_triple_persistent(DevScene s0, PassParams prm, float4* __restrict__ res,
                                                            int* __restrict__ pixelCounter, int threshold)
 {
     const int lane = threadIdx.x & 63;
     Counters cn;
     int region = (int)(blockIdx.x % kRegions);
     int exhausted = 0;
-    const int W = s.filmW, H = s.filmH;
+    const int W = s0.filmW, H = s0.filmH;
     const int nIndex = ((W + 7) / 8) * ((H + 7) / 8) * 64;
     const int nUnits = nIndex * prm.nPasses;
 
@@ -514,6 +514,7 @@ __global__ __launch_bounds__(256, 4) void k_triple_persistent(DevScene s, PassPa
     wk.dist = 0.f; wk.k = 0; wk.closest = -1; wk.any = false;
 
     for (;;) {
+        const DevScene& s = sceneArg(s0);
         if (!walking) {
             if (phase == kPrimary) {
                 const int hit = wk.closest;
