@@ -374,8 +374,8 @@ ZGL_D vec3 replayPath(const DevScene& s, const Replay& rp, const PassParams& prm
         if (prm.russianRoulette) {
             const bool survive = !(sample1D<false>(s, rng, cn) >= rrProb);
             const vec3 tpS = throughput / rrProb, skS = tpSkip / rrProb;
-            throughput = survive ? tpS : throughput;
-            tpSkip = survive ? skS : tpSkip;
+            throughput = select3(survive, tpS, throughput);
+            tpSkip = select3(survive, skS, tpSkip);
             if (!survive) break;
         }
     }

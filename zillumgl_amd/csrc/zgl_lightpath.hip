@@ -214,7 +214,7 @@ __global__ __launch_bounds__(256, 4) void k_lightpath_persistent(DevScene s0, Pa
                         const float continueProb = fminG(maxComponent(bs.bsdf / bsdfPdf), 1.0f);
                         const bool survive = !(sample1D<false>(s, rng, cn) >= continueProb);
                         const vec3 scaled = throughput / continueProb;
-                        throughput = survive ? scaled : throughput;
+                        throughput = select3(survive, scaled, throughput);
                         alive = alive && survive;
                     }
                     if (TRIPLE) {

@@ -186,6 +186,12 @@ ZGL_HD vec3 operator/(vec3 a, float s) { return v3(a.x / s, a.y / s, a.z / s); }
 ZGL_HD vec3 operator+(vec3 a, float s) { return v3(a.x + s, a.y + s, a.z + s); }
 ZGL_HD vec3 operator-(vec3 a, float s) { return v3(a.x - s, a.y - s, a.z - s); }
 ZGL_HD vec3 operator-(vec3 a) { return v3(-a.x, -a.y, -a.z); }
+// Per-component select.  A `c ? a : b` on the struct itself can be lowered to a pick
+// between two stack copies (scratch memory); three float selects stay in registers.
+ZGL_HD vec3 select3(bool c, vec3 a, vec3 b)
+{
+    return v3(c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z);
+}
 ZGL_HD vec3& operator+=(vec3& a, vec3 b) { a = a + b; return a; }
 ZGL_HD vec3& operator*=(vec3& a, vec3 b) { a = a * b; return a; }
 ZGL_HD vec3& operator*=(vec3& a, float s) { a = a * s; return a; }

@@ -187,18 +187,20 @@ __device__ __forceinline__ void naiveBody(const DevScene& s0, const PassParams& 
                     result += radiance * throughput * weight;
                     phase = kFinish;
                 } else {
-                    phase = kSurface;
+                    // the next vertex's state is set before the roulette, and the roulette
+                    // is written as selects: state a surviving path set after a roulette
+                    // branch has been compiled wrongly (the global-queue tracer)
+                    wk.closest = nextId;
+                    pos = nextPos;
+                    wo = -wi;
+                    bool survive = true;
                     if (prm.russianRoulette) {
-                        if (sample1D<C>(s, rng, cn) >= rrProb) phase = kFinish;
-                        else throughput /= rrProb;
+                        survive = !(sample1D<C>(s, rng, cn) >= rrProb);
+                        const vec3 scaled = throughput / rrProb;
+                        throughput = select3(survive, scaled, throughput);
                     }
-                    if (phase == kSurface) {
-                        wk.closest = nextId;
-                        pos = nextPos;
-                        wo = -wi;
-                        bounce++;
-                        if (bounce > prm.maxDepth) phase = kFinish;
-                    }
+                    bounce++;
+                    phase = (survive && bounce <= prm.maxDepth) ? kSurface : kFinish;
                 }
             }
 
@@ -571,21 +573,22 @@ __global__ __launch_bounds__(256, 4) void k_triple_persistent(DevScene s0, PassP
                     result += lightLe<false>(s, lightId, nextPos, -wi, cn) * throughput * weight;
                     phase = kFinish;
                 } else {
-                    phase = kSurface;
+                    // set before the roulette, which is written as selects (see naiveBody)
+                    pos = nextPos;
+                    wo = -wi;
+                    id = nextId;
+                    bool survive = true;
                     if (prm.russianRoulette) {
-                        if (sample1D<false>(s, rng, cn) >= rrProb) phase = kFinish;
-                        else throughput /= rrProb;
+                        survive = !(sample1D<false>(s, rng, cn) >= rrProb);
+                        const vec3 scaled = throughput / rrProb;
+                        throughput = select3(survive, scaled, throughput);
                     }
-                    if (phase == kSurface) {
-                        const float coef = cPrev / remapP(pdfNext * absDot(triangleNormalShad<false>(s, nextId, nextPos, cn), wi));
-                        t1s0 *= coef;
-                        t1s1 *= coef;
-                        pos = nextPos;
-                        wo = -wi;
-                        id = nextId;
-                        bounce++;
-                        if (bounce > prm.maxDepth) phase = kFinish;
-                    }
+                    // (also on a path that ends here, which does not read them again)
+                    const float coef = cPrev / remapP(pdfNext * absDot(triangleNormalShad<false>(s, nextId, nextPos, cn), wi));
+                    t1s0 *= coef;
+                    t1s1 *= coef;
+                    bounce++;
+                    phase = (survive && bounce <= prm.maxDepth) ? kSurface : kFinish;
                 }
             }
 

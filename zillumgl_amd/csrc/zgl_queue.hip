@@ -199,7 +199,7 @@ __device__ bool extendPath(const DevScene& s, const PassParams& prm, const Queue
         const float u = sample1D<false>(s, rng, cn);
         survive = !(u >= continueProb);
         const vec3 scaled = throughput / continueProb;
-        throughput = survive ? scaled : throughput;
+        throughput = select3(survive, scaled, throughput);
     }
     out = makeItem(nextPos, throughput, -wi, pix, nextId);
     return survive;
