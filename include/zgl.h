@@ -170,8 +170,13 @@ typedef struct zgl_pass_params {
     int32_t blocks_one_pass;      /* light: threadBlocksOnePass; triple: LPTBlocksOnePass */
     int32_t loops_per_pass;       /* triple: LPTLoopsPerPass */
     int32_t count_bytes;          /* accumulate algorithmic-byte counters (slower) */
-    int32_t regen_threshold;      /* naive: lanes of a wave64 still walking the MTBVH when the
-                                     wave stops to shade and reissue rays (1..63; 0 = default 20) */
+    int32_t regen_threshold;      /* naive, light, triple (persistent kernels): lanes of a wave64
+                                     still walking the MTBVH when the wave stops to shade and
+                                     reissue rays (1..63).  0 = default: from the triangle count
+                                     n, round(1.4 log2 n - 2) clamped to 4..28 for the naive
+                                     tracer, round(1.3 log2 n - 7) clamped to 1..24 for the
+                                     triple tracer's camera pass, 4 for light paths.  Frames do
+                                     not depend on it. */
 } zgl_pass_params;
 /* ZGL_INTEGRATOR_NAIVE keeps each sample's value (16 bytes) in a context-owned buffer of
  * at most 4 GiB and adds them to the frame in pass order; calls with more passes than fit

@@ -18,8 +18,26 @@
 
 using namespace zgl;
 
-// Lanes still walking when a wave of the persistent schedule stops to shade.
+// Lanes still walking when a wave of the persistent schedule stops to shade: the
+// batch's default.
 constexpr int kDefaultRegenThreshold = 20;
+
+// zgl_render's default for it (regen_threshold 0).  Frames do not depend on it; the
+// fastest value grows with the length of the walks, which grows with log2 of the
+// triangle count.  Fitted to sweeps on one MI355X over scenes of 36 to 1.3M triangles:
+// the naive tracer was fastest at 4 with 36 triangles, 12 with 852, 16-20 with 10k-100k
+// and 24-28 with 0.8M-1.3M; the triple tracer's camera pass at 1, 8-12 and 16 with 36,
+// 10k and 100k; the light paths at 1-4 at every size.
+enum class WalkKind { Naive, TripleCamera, LightPaths };
+static int defaultThreshold(WalkKind kind, int bvhSize)
+{
+    const double l = std::log2((double)std::max(2, (bvhSize + 1) / 2));
+    switch (kind) {
+    case WalkKind::LightPaths: return 4;
+    case WalkKind::TripleCamera: return std::min(24, std::max(1, (int)std::lround(1.3 * l - 7.0)));
+    default: return std::min(28, std::max(4, (int)std::lround(1.4 * l - 2.0)));
+    }
+}
 // Largest sample-value plus traversal-log buffer of one adjoint chunk of passes.
 constexpr size_t kAdjointLogBytes = (size_t)8 << 30;
 // Largest total of a batch's per-sample value buffers.
@@ -755,7 +773,8 @@ zgl_status zgl_render(zgl_context* ctx, const zgl_pass_params* p)
         float* film = (float*)ctx->film.p;
         if (p->integrator == ZGL_INTEGRATOR_NAIVE) {
             if (!ctx->naiveBlocksPerCU) ctx->naiveBlocksPerCU = naivePersistentBlocksPerCU();
-            const int threshold = p->regen_threshold > 0 ? p->regen_threshold : kDefaultRegenThreshold;
+            const int threshold =
+                p->regen_threshold > 0 ? p->regen_threshold : defaultThreshold(WalkKind::Naive, s.bvhSize);
             // per-sample values of up to 4 GB worth of passes (121 passes at 1080p)
             const size_t perPass = (size_t)ctx->filmW * ctx->filmH * 16;
             const int resPasses = (int)std::max<size_t>(1, std::min<size_t>(pp.nPasses, ((size_t)4 << 30) / perPass));
@@ -786,7 +805,8 @@ zgl_status zgl_render(zgl_context* ctx, const zgl_pass_params* p)
                 hipCheck(launchLight(s, pp, film, cnt, ctx->stream), "light pass");
             } else {
                 if (!ctx->lightBlocksPerCU) ctx->lightBlocksPerCU = lightPersistentBlocksPerCU(false);
-                const int threshold = p->regen_threshold > 0 ? p->regen_threshold : kDefaultRegenThreshold;
+                const int threshold =
+                    p->regen_threshold > 0 ? p->regen_threshold : defaultThreshold(WalkKind::LightPaths, s.bvhSize);
                 hipCheck(launchLightPersistent(s, pp, false, film, (int*)ctx->workCounter.p, threshold,
                                                ctx->numCUs * ctx->lightBlocksPerCU, ctx->stream),
                          "light pass");
@@ -804,7 +824,8 @@ zgl_status zgl_render(zgl_context* ctx, const zgl_pass_params* p)
                 const int resPasses =
                     (int)std::max<size_t>(1, std::min<size_t>(pp.nPasses, ((size_t)4 << 30) / perPass));
                 ctx->ensure(ctx->naiveRes, perPass * resPasses, "sample values");
-                const int threshold = p->regen_threshold > 0 ? p->regen_threshold : kDefaultRegenThreshold;
+                const int threshold =
+                    p->regen_threshold > 0 ? p->regen_threshold : defaultThreshold(WalkKind::TripleCamera, s.bvhSize);
                 hipCheck(launchTriplePersistent(s, pp, film, (float4*)ctx->naiveRes.p, resPasses,
                                                 (int*)ctx->workCounter.p, threshold,
                                                 ctx->numCUs * ctx->tripleBlocksPerCU, ctx->stream),
@@ -816,7 +837,8 @@ zgl_status zgl_render(zgl_context* ctx, const zgl_pass_params* p)
                 hipCheck(launchTripleLPT(sl, pp, film, cnt, ctx->stream), "triple light pass");
             } else {
                 if (!ctx->tripleLightBlocksPerCU) ctx->tripleLightBlocksPerCU = lightPersistentBlocksPerCU(true);
-                const int threshold = p->regen_threshold > 0 ? p->regen_threshold : kDefaultRegenThreshold;
+                const int threshold =
+                    p->regen_threshold > 0 ? p->regen_threshold : defaultThreshold(WalkKind::LightPaths, s.bvhSize);
                 hipCheck(launchLightPersistent(sl, pp, true, film, (int*)ctx->workCounter.p, threshold,
                                                ctx->numCUs * ctx->tripleLightBlocksPerCU, ctx->stream),
                          "triple light pass");
@@ -845,7 +867,8 @@ zgl_status zgl_render_static(zgl_context* ctx, const zgl_pass_params* p)
         const int chunk = (int)std::min<size_t>({(size_t)p->n_passes, kStaticResBytes / perPass,
                                                  (size_t)kStaticChunkPasses});
         if (!ctx->naiveBlocksPerCU) ctx->naiveBlocksPerCU = naivePersistentBlocksPerCU();
-        const int threshold = p->regen_threshold > 0 ? p->regen_threshold : kDefaultRegenThreshold;
+        const int threshold =
+            p->regen_threshold > 0 ? p->regen_threshold : defaultThreshold(WalkKind::Naive, ctx->scene.bvhSize);
         int* counter = reinterpret_cast<int*>(base);
         uint32_t* sobol = reinterpret_cast<uint32_t*>(base + kStaticSobolOffset);
         float4* res = reinterpret_cast<float4*>(base + kStaticResOffset);
@@ -950,7 +973,8 @@ zgl_status zgl_adjoint(zgl_context* ctx, const zgl_pass_params* p, const float* 
             q.freeCounter = pp.freeCounter + k0;
             q.nPasses = std::min(chunk, pp.nPasses - k0);
             hipCheck(launchNaiveLogged(s, q, (float4*)ctx->naiveRes.p, (int2*)ctx->hitLog.p, (int*)ctx->workCounter.p,
-                                       p->regen_threshold > 0 ? p->regen_threshold : kDefaultRegenThreshold,
+                                       p->regen_threshold > 0 ? p->regen_threshold
+                                                              : defaultThreshold(WalkKind::Naive, s.bvhSize),
                                        ctx->numCUs * ctx->naiveBlocksPerCU, ctx->stream),
                      "logged forward pass");
             hipCheck(launchNaiveReplay(s, q, (const float4*)ctx->evalMat.p, (const float4*)ctx->evalEnv.p,

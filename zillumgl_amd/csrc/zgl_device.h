@@ -212,6 +212,28 @@ struct Walk {
     int leaf;  // primitive of the leaf the walk is stopped at, -1 while it moves
     bool any;
 };
+// The row table and the triangle positions as seen by a walk loop: loaded from the
+// scene once before the loop and held in scalar registers.  Read through the scene at
+// every step, the compiler reloads the pointer from the kernel arguments inside each
+// row step, which puts a scalar load and its wait on every row's dependent chain; the
+// empty asm keeps it from doing so.  The pointers keep the global address space, so the
+// loads stay global loads.
+typedef float GFloat4 __attribute__((ext_vector_type(4)));
+typedef const GFloat4 __attribute__((address_space(1)))* GRows;
+typedef const float __attribute__((address_space(1)))* GFloats;
+struct WalkMem {
+    GRows rows;
+    GRows tris;
+};
+ZGL_D WalkMem walkMem(const DevScene& s)
+{
+    WalkMem m{(GRows)s.rows, (GRows)s.tris};
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" : "+s"(m.rows), "+s"(m.tris));
+#endif
+    return m;
+}
+
 template <bool C>
 ZGL_D void walkStart(const DevScene& s, Walk& w, const Ray& r, bool any, float tmax, Counters& cn)
 {
@@ -227,11 +249,11 @@ ZGL_D void walkStart(const DevScene& s, Walk& w, const Ray& r, bool any, float t
 // One row: returns true when the walk has finished.  A leaf whose box is hit stops the
 // walk (w.leaf >= 0) without testing the triangle.  `general` is true when the caller
 // knows that no lane walking with it has a special ray.
-template <bool C> ZGL_D bool walkRow(const DevScene& s, Walk& w, bool general, Counters& cn)
+template <bool C> ZGL_D bool walkRow(const WalkMem& m, Walk& w, bool general, Counters& cn)
 {
-    const float4* row = rowPtr(s, w.k);
-    const float4 lo = row[0];
-    const float4 hi = row[1];
+    const GRows row = m.rows + (size_t)(uint32_t)w.k * 2;
+    const GFloat4 lo = row[0];
+    const GFloat4 hi = row[1];
     if (C) { cn.rows++; cn.bytes += 36; }
     const vec3 bMin = v3(lo.x, lo.y, lo.z), bMax = v3(lo.w, hi.x, hi.y);
     // the three-slab test straight after its loads; the other cases only when a lane of
@@ -249,12 +271,14 @@ template <bool C> ZGL_D bool walkRow(const DevScene& s, Walk& w, bool general, C
 }
 // The triangle test of the leaf the walk is stopped at; returns true when the walk has
 // finished.
-template <bool C> ZGL_D bool walkLeaf(const DevScene& s, Walk& w, Counters& cn)
+template <bool C> ZGL_D bool walkLeaf(const WalkMem& m, Walk& w, Counters& cn)
 {
     const int prim = w.leaf;
     w.leaf = -1;
-    vec3 a, b, c;
-    loadTri(s, prim, a, b, c);
+    const GRows tri = m.tris + (size_t)prim * 3;
+    const GFloat4 t0 = tri[0], t1 = tri[1];
+    const float t2x = ((GFloats)(tri + 2))[0];
+    const vec3 a = v3(t0.x, t0.y, t0.z), b = v3(t0.w, t1.x, t1.y), c = v3(t1.z, t1.w, t2x);
     if (C) { cn.tests++; cn.bytes += 48; }
     float t;
     if (intersectTriangle(a, b, c, w.p.o, w.p.d, t) && t < w.dist) {

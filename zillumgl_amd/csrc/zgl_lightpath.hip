@@ -268,15 +268,16 @@ __global__ __launch_bounds__(256, 4) void k_lightpath_persistent(DevScene s0, Pa
         // walking: up to kRowsPerRound rows per lane per round, leaves tested together once
         // per round; a finished connection walk splats and starts its extension walk here
         bool general = __ballot(walking && wk.p.special) == 0ull;
+        const WalkMem wm = walkMem(s);
         for (;;) {
 #pragma unroll
             for (int u = 0; u < kRowsPerRound; ++u) {
                 if (walking && wk.leaf < 0) {
-                    if (walkRow<false>(s, wk, general, cn)) walking = false;
+                    if (walkRow<false>(wm, wk, general, cn)) walking = false;
                 }
             }
             if (walking && wk.leaf >= 0) {
-                if (walkLeaf<false>(s, wk, cn)) walking = false;
+                if (walkLeaf<false>(wm, wk, cn)) walking = false;
             }
             if (!walking && phase == kConnect) settle();
             general = __ballot(walking && wk.p.special) == 0ull;

@@ -393,15 +393,16 @@ __device__ __forceinline__ void naiveBody(const DevScene& s0, const PassParams& 
         // shading step would do), instead of idling until the wave leaves the loop.
         {
             bool general = __ballot(walking && wk.p.special) == 0ull;
+            const WalkMem wm = walkMem(s);
             for (;;) {
 #pragma unroll
                 for (int u = 0; u < kRowsPerRound; ++u) {
                     if (walking && wk.leaf < 0) {
-                        if (walkRow<C>(s, wk, general, cn)) walking = false;
+                        if (walkRow<C>(wm, wk, general, cn)) walking = false;
                     }
                 }
                 if (walking && wk.leaf >= 0) {
-                    if (walkLeaf<C>(s, wk, cn)) walking = false;
+                    if (walkLeaf<C>(wm, wk, cn)) walking = false;
                 }
                 // a shadow walk that ended settles its NEE term and starts the extension walk
                 if (!walking && phase == kShadow) {
@@ -724,15 +725,16 @@ __global__ __launch_bounds__(256, 4) void k_triple_persistent(DevScene s0, PassP
 
         {
             bool general = __ballot(walking && wk.p.special) == 0ull;
+            const WalkMem wm = walkMem(s);
             for (;;) {
 #pragma unroll
                 for (int u = 0; u < kRowsPerRound; ++u) {
                     if (walking && wk.leaf < 0) {
-                        if (walkRow<false>(s, wk, general, cn)) walking = false;
+                        if (walkRow<false>(wm, wk, general, cn)) walking = false;
                     }
                 }
                 if (walking && wk.leaf >= 0) {
-                    if (walkLeaf<false>(s, wk, cn)) walking = false;
+                    if (walkLeaf<false>(wm, wk, cn)) walking = false;
                 }
                 // a visibility walk that ended settles its s=1 term and starts the extension
                 if (!walking && phase == kShadow) {

@@ -212,14 +212,15 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_trace(DevScene s, const WfRay* 
             continue;
         }
         const bool general = __ballot(walking && wk.p.special) == 0ull;
+        const WalkMem wm = walkMem(s);
 #pragma unroll
         for (int u = 0; u < kWfRows; ++u) {
             if (walking && wk.leaf < 0) {
-                if (walkRow<false>(s, wk, general, cn)) walking = false;
+                if (walkRow<false>(wm, wk, general, cn)) walking = false;
             }
         }
         if (walking && wk.leaf >= 0) {
-            if (walkLeaf<false>(s, wk, cn)) walking = false;
+            if (walkLeaf<false>(wm, wk, cn)) walking = false;
         }
         // a walk that ended stores its result in its path
         if (!walking && own) {
