@@ -276,14 +276,16 @@ zgl_status zgl_batch_set(zgl_batch* batch, zgl_context* const* contexts, const z
  * (any output may be NULL). */
 zgl_status zgl_batch_plan(const int32_t* film_wh, const zgl_pass_params* params, int32_t n, int64_t* sample_bytes,
                           int32_t* sobol_first, int32_t* sobol_count);
-/* regen_threshold 1..63 (0 = default 20); blocks_per_cu 1..max (0 = max resident);
+/* regen_threshold 1..63 for every scene, or 0 (the default): each scene's own zgl_render
+ * default for the naive tracer; blocks_per_cu 1..max (0 = max resident);
  * order: a permutation of 0..n-1 giving the launch order (NULL keeps the current one). */
 zgl_status zgl_batch_configure(zgl_batch* batch, int32_t regen_threshold, int32_t blocks_per_cu,
                                const int32_t* order);
 zgl_status zgl_batch_launch(zgl_batch* batch);
 zgl_status zgl_batch_synchronize(zgl_batch* batch);
 /* Autotuner: times the batch (median of `reps` launches after a warm-up) over scene
- * orders (as given / most work first), regeneration thresholds {8, ..., 32} and the two
+ * orders (as given / most work first), regeneration thresholds {per scene (0), 8, ..., 32}
+ * and the two
  * highest blocks-per-CU counts, coordinate by coordinate, and keeps the fastest.  It
  * launches the batch repeatedly: frames of passes that do not start at pass 0
  * accumulate every launch. */

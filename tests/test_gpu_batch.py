@@ -124,7 +124,7 @@ def test_batch_tune_keeps_results(contexts, batch_scenes):
     b.set(contexts, passes)
     best = b.tune(reps=1)
     cfg = b.config()
-    assert best > 0 and 1 <= cfg["regen_threshold"] <= 63 and sorted(cfg["order"]) == list(range(6))
+    assert best > 0 and 0 <= cfg["regen_threshold"] <= 63 and sorted(cfg["order"]) == list(range(6))
     frames = _batch_frames(b, contexts)
     for k, (f, r) in enumerate(zip(frames, refs)):
         assert np.array_equal(f.view(np.uint32), r.view(np.uint32)), k

@@ -18,10 +18,6 @@
 
 using namespace zgl;
 
-// Lanes still walking when a wave of the persistent schedule stops to shade: the
-// batch's default.
-constexpr int kDefaultRegenThreshold = 20;
-
 // zgl_render's default for it (regen_threshold 0).  Frames do not depend on it; the
 // fastest value grows with the length of the walks, which grows with log2 of the
 // triangle count.  Fitted to sweeps on one MI355X over scenes of 36 to 1.3M triangles:
@@ -201,7 +197,7 @@ struct zgl_batch {
     std::vector<size_t> resOffset;  // per context (index in ctxs): float4 offset into res
     int maxPixels = 0;
     int numCUs = 0, maxBlocksPerCU = 1;
-    int threshold = kDefaultRegenThreshold, blocksPerCU = 0;
+    int threshold = 0, blocksPerCU = 0;  // threshold 0: each scene's zgl_render default
     int sobolFirst = 0, sobolCount = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -1169,6 +1165,7 @@ static void batchUpload(zgl_batch* b)
         const zgl_pass_params* p = &b->params[i];
         BatchScene& it = b->host[k];
         it.s = ctx->scene;
+        it.threshold = b->threshold > 0 ? b->threshold : defaultThreshold(WalkKind::Naive, it.s.bvhSize);
         it.s.sobolTab = (const uint32_t*)b->sobolTab.p;
         it.s.sobolTabFirst = b->sobolFirst;
         it.s.sobolTabCount = b->sobolCount;
@@ -1255,7 +1252,7 @@ zgl_status zgl_batch_configure(zgl_batch* b, int32_t regen_threshold, int32_t bl
             }
             b->order.assign(order, order + n);
         }
-        b->threshold = regen_threshold > 0 ? regen_threshold : kDefaultRegenThreshold;
+        b->threshold = regen_threshold;
         b->blocksPerCU = blocks_per_cu > 0 ? blocks_per_cu : b->maxBlocksPerCU;
         batchUpload(b);
     });
@@ -1294,7 +1291,7 @@ zgl_status zgl_batch_launch(zgl_batch* b)
             // the whole launch (counter reset, persistent kernel, accumulation) as one graph
             hipCheck(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "capture");
             const hipError_t e = launchNaiveBatch((const BatchScene*)b->items.p, n, b->maxPixels, (int*)b->counters.p,
-                                                  b->threshold, b->numCUs * b->blocksPerCU, b->stream);
+                                                  b->numCUs * b->blocksPerCU, b->stream);
             hipGraph_t g = nullptr;
             const hipError_t e2 = hipStreamEndCapture(b->stream, &g);
             hipCheck(e, "batch capture");
@@ -1361,7 +1358,7 @@ zgl_status zgl_batch_tune(zgl_batch* b, int32_t reps, double* best_ms)
             const double t = batchTime(b, reps);
             if (t < best) { best = t; ord = o; }
         }
-        for (int t : {8, 12, 16, 20, 24, 32}) {
+        for (int t : {0, 8, 12, 16, 20, 24, 32}) {  // 0: each scene's own default
             config(t, bpc, ord);
             const double tm = batchTime(b, reps);
             if (tm < best) { best = tm; th = t; }

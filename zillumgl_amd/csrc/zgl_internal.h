@@ -34,7 +34,8 @@ struct BatchScene {
     DevScene s;
     PassParams prm;
     float4* frame;
-    float4* res;  // prm.nPasses x filmW x filmH sample values
+    float4* res;    // prm.nPasses x filmW x filmH sample values
+    int threshold;  // lanes still walking when a wave working on this scene stops to shade
 };
 
 hipError_t launchSobolTable(const uint32_t* sobolM, uint32_t* tab, int first, int count, hipStream_t st);
@@ -67,7 +68,7 @@ hipError_t launchNaiveLogged(const DevScene& s, const PassParams& p, float4* res
                              int threshold, int blocks, hipStream_t st);
 int naiveBatchBlocksPerCU();
 // maxPixels: the largest filmW x filmH of the batch (sizes the accumulate launch).
-hipError_t launchNaiveBatch(const BatchScene* items, int nScenes, int maxPixels, int* sceneCounters, int threshold,
+hipError_t launchNaiveBatch(const BatchScene* items, int nScenes, int maxPixels, int* sceneCounters,
                             int blocks, hipStream_t st);
 // Differentiable naive estimator: dFrame == null renders the replay estimator into
 // frame; otherwise accumulates FP64 gradients into dMat (nMaterials x 5) and the envCopies

@@ -73,7 +73,8 @@ template <bool C> __device__ void flushCountersP(const Counters& cn, unsigned lo
 // {1 if occluded else 0, 0} for shadow queries.  The adjoint replays read it.
 // BATCH = true: the scenes of `items` in order; a wave works on one scene at a time
 // (scene data stays wave-uniform), takes its samples from that scene's counter, and
-// moves to the next scene once its lanes have finished the samples they hold.
+// moves to the next scene once its lanes have finished the samples they hold; its
+// regeneration threshold is that scene's (items[].threshold).
 template <bool C, bool BATCH, bool LOG>
 __device__ __forceinline__ void naiveBody(const DevScene& s0, const PassParams& prm0, float4* res0, int* counter0,
                                           const BatchScene* items, int nScenes, int* sceneCounters, int threshold,
@@ -110,6 +111,7 @@ __device__ __forceinline__ void naiveBody(const DevScene& s0, const PassParams& 
         const PassParams& prm = BATCH ? items[csu].prm : prm0;
         float4* const res = BATCH ? items[csu].res : res0;
         int* const pixelCounter = BATCH ? sceneCounters + csu : counter0;
+        const int sceneThreshold = BATCH ? items[csu].threshold : threshold;
         const int W = s.filmW, H = s.filmH;
         const int nIndex = ((W + 7) / 8) * ((H + 7) / 8) * 64;
         // work units: (tile, pass, pixel of the tile)
@@ -420,7 +422,7 @@ __device__ __forceinline__ void naiveBody(const DevScene& s0, const PassParams& 
                     }
                 }
                 general = __ballot(walking && wk.p.special) == 0ull;
-                if (__popcll(__ballot(walking)) <= threshold) break;
+                if (__popcll(__ballot(walking)) <= sceneThreshold) break;
             }
         }
     }
@@ -473,10 +475,10 @@ __global__ __launch_bounds__(256) void k_accumulate_batch(const BatchScene* __re
 }
 
 __global__ __launch_bounds__(256, 4) void k_naive_batch(const BatchScene* __restrict__ items, int nScenes,
-                                                     int* __restrict__ sceneCounters, int threshold)
+                                                     int* __restrict__ sceneCounters)
 {
-    naiveBody<false, true, false>(items[0].s, items[0].prm, nullptr, nullptr, items, nScenes, sceneCounters,
-                                  threshold, nullptr, nullptr);
+    naiveBody<false, true, false>(items[0].s, items[0].prm, nullptr, nullptr, items, nScenes, sceneCounters, 0,
+                                  nullptr, nullptr);
 }
 
 // ---------------------------------------------------------------------------
@@ -822,13 +824,11 @@ hipError_t launchNaiveLogged(const DevScene& s, const PassParams& p, float4* res
     return hipGetLastError();
 }
 
-hipError_t launchNaiveBatch(const BatchScene* items, int nScenes, int maxPixels, int* sceneCounters, int threshold,
-                            int blocks, hipStream_t st)
+hipError_t launchNaiveBatch(const BatchScene* items, int nScenes, int maxPixels, int* sceneCounters, int blocks,
+                            hipStream_t st)
 {
-    if (threshold < 0) threshold = 0;
-    if (threshold > 63) threshold = 63;
     hipLaunchKernelGGL(k_reset_counters, dim3((nScenes + 255) / 256), dim3(256), 0, st, sceneCounters, nScenes);
-    hipLaunchKernelGGL(k_naive_batch, dim3(blocks), dim3(256), 0, st, items, nScenes, sceneCounters, threshold);
+    hipLaunchKernelGGL(k_naive_batch, dim3(blocks), dim3(256), 0, st, items, nScenes, sceneCounters);
     hipLaunchKernelGGL(k_accumulate_batch, dim3((unsigned)((maxPixels + 255) / 256), (unsigned)std::min(nScenes, 4096)),
                        dim3(256), 0, st, items, nScenes);
     return hipGetLastError();
