@@ -1220,7 +1220,7 @@ zgl_status zgl_batch_set(zgl_batch* b, zgl_context* const* ctxs, const zgl_pass_
         hipCheck(launchSobolTable((const uint32_t*)ctxs[0]->sobolM.p, (uint32_t*)b->sobolTab.p, b->sobolFirst,
                                   b->sobolCount, b->stream),
                  "sobol table");
-        b->ensure(b->counters, (size_t)n * 4, "batch counters");
+        b->ensure(b->counters, (size_t)n * kUnitRegions * 4, "batch counters");
         // every scene's sample values (n_passes x W x H float4), added to its frame in
         // pass order after the persistent kernel
         b->resOffset.resize(n);

@@ -30,6 +30,10 @@ struct PassParams {
 constexpr int NumCounters = 8;
 
 // One scene of a batched persistent launch (device-resident table).
+// Unit regions of the persistent schedules, one per XCD: each has its own counter (a
+// batch has kUnitRegions counters per scene).
+constexpr int kUnitRegions = 8;
+
 struct BatchScene {
     DevScene s;
     PassParams prm;

@@ -37,7 +37,7 @@ enum : int {
 // MTBVH rows each walking lane advances per round of the walking loop (one leaf test
 // and one ballot per round).
 constexpr int kRowsPerRound = 4;
-constexpr int kRegions = 8;  // pixel regions, one per XCD
+constexpr int kRegions = kUnitRegions;  // unit regions, one per XCD
 
 // Pixel order for the dynamic fetch: 8x8 tiles in row-major tile order.
 __device__ __forceinline__ void pixelOfIndex(int idx, int W, int& x, int& y)
@@ -84,7 +84,7 @@ __device__ __forceinline__ void naiveBody(const DevScene& s0, const PassParams& 
     Counters cn;
     int cs = 0;
     int region = (int)(blockIdx.x % kRegions);  // the workgroup's XCD under round-robin dispatch
-    int exhausted = 0;  // batch: the wave's current scene (the same in every lane)
+    int exhausted = 0;  // regions of the current scene found used up in a row (the same in every lane)
 
     int phase = kNewPixel;
     int pix = -1, pass = 0;
@@ -110,7 +110,7 @@ __device__ __forceinline__ void naiveBody(const DevScene& s0, const PassParams& 
         const DevScene& s = BATCH ? items[csu].s : sceneArg(s0);
         const PassParams& prm = BATCH ? items[csu].prm : prm0;
         float4* const res = BATCH ? items[csu].res : res0;
-        int* const pixelCounter = BATCH ? sceneCounters + csu : counter0;
+        int* const pixelCounter = BATCH ? sceneCounters + csu * kRegions : counter0;
         const int sceneThreshold = BATCH ? items[csu].threshold : threshold;
         const int W = s.filmW, H = s.filmH;
         const int nIndex = ((W + 7) / 8) * ((H + 7) / 8) * 64;
@@ -311,37 +311,32 @@ __device__ __forceinline__ void naiveBody(const DevScene& s0, const PassParams& 
         // pixels are handed out per wave: one atomic for all lanes that need one.  One
         // region of 8x8 tiles per XCD (workgroups are dispatched to XCDs round-robin), so
         // that the waves sharing an L2 start with neighbouring pixels; a wave whose region
-        // is used up moves on to the next, and stops once all eight are.
+        // is used up moves on to the next, and once all eight are, stops (one scene) or
+        // waits to move to the next scene (batch).
         const unsigned long long need = __ballot(!walking && phase == kNewPixel);
         if (need) {
             const int n = __popcll(need);
             const int leader = __ffsll((long long)need) - 1;
             int base = 0;
-            int limit = nUnits;
-            if (BATCH) {
-                if (lane == leader) base = atomicAdd(pixelCounter, n);
-                base = __shfl(base, leader);
+            // the wave's region of the image first (one per XCD), then the others
+            const int len = ((nUnits / kRegions + 63) / 64) * 64;
+            const int start = min(nUnits, region * len);
+            const int limit = min(nUnits, start + len);
+            if (lane == leader) base = atomicAdd(pixelCounter + region, n);
+            base = __shfl(base, leader) + start;
+            if (base >= limit) {
+                region = (region + 1) % kRegions;
+                exhausted++;
             } else {
-                // the wave's region of the image first (one per XCD), then the others
-                const int len = ((nUnits / kRegions + 63) / 64) * 64;
-                const int start = min(nUnits, region * len);
-                limit = min(nUnits, start + len);
-                if (lane == leader) base = atomicAdd(pixelCounter + region, n);
-                base = __shfl(base, leader) + start;
-                if (base >= limit) {
-                    region = (region + 1) % kRegions;
-                    exhausted++;
-                } else {
-                    exhausted = 0;
-                }
+                exhausted = 0;
             }
             if (!walking && phase == kNewPixel) {
                 const unsigned long long below = need & ((1ull << lane) - 1ull);
                 const int idx = base + __popcll(below);
                 pix = -1;
-                if (BATCH ? idx >= nUnits : exhausted >= kRegions) {
+                if (exhausted >= kRegions) {
                     phase = BATCH ? kWait : kDone;
-                } else if (!BATCH && idx >= limit) {
+                } else if (idx >= limit) {
                     // this region is used up: fetch again from the next one
                 } else {
                     const int group = idx >> 6;
@@ -379,6 +374,8 @@ __device__ __forceinline__ void naiveBody(const DevScene& s0, const PassParams& 
             // every lane idle and some waiting: move the wave to the next scene
             if (__ballot(walking || (phase != kWait && phase != kDone)) == 0ull && __ballot(phase == kWait) != 0ull) {
                 cs = csu + 1;
+                region = (int)(blockIdx.x % kRegions);
+                exhausted = 0;
                 if (phase == kWait) phase = (cs < nScenes) ? kNewPixel : kDone;
                 continue;
             }
@@ -827,7 +824,8 @@ hipError_t launchNaiveLogged(const DevScene& s, const PassParams& p, float4* res
 hipError_t launchNaiveBatch(const BatchScene* items, int nScenes, int maxPixels, int* sceneCounters, int blocks,
                             hipStream_t st)
 {
-    hipLaunchKernelGGL(k_reset_counters, dim3((nScenes + 255) / 256), dim3(256), 0, st, sceneCounters, nScenes);
+    const int nCounters = nScenes * kRegions;
+    hipLaunchKernelGGL(k_reset_counters, dim3((nCounters + 255) / 256), dim3(256), 0, st, sceneCounters, nCounters);
     hipLaunchKernelGGL(k_naive_batch, dim3(blocks), dim3(256), 0, st, items, nScenes, sceneCounters);
     hipLaunchKernelGGL(k_accumulate_batch, dim3((unsigned)((maxPixels + 255) / 256), (unsigned)std::min(nScenes, 4096)),
                        dim3(256), 0, st, items, nScenes);
