@@ -324,12 +324,15 @@ def bvh_bench(reps):
             "levels": st["levels"], "one_wave_subtrees": st["small_subtrees"], "depth": st["depth"]}
 
 
-def rmse_vs_converged(ctx, W, H, spp, depth, ref_spp):
+def rmse_vs_converged(ctx, W, H, spp, depth, ref_spp, precision="f64"):
     """RMSE of the benchmark image (mean of passes 0..spp-1) against a converged image of
     ref_spp further passes (spp..spp+ref_spp-1, independent of the image's samples) of
-    the same estimator, summed per launch of 64 passes into FP64 on the host (each
-    launch's frame starts from zero).  The reference's own noise adds spp / ref_spp of
-    the image's variance to the squared RMSE; the value is corrected for it."""
+    the same estimator: by default rendered in FP64 (zgl_render_reference_f64, the FP64
+    instance of the device library, equal to the oracle's FP64 mode per pixel,
+    tests/test_gpu_reference64.py), or with precision "f32" by the FP32 kernel, summed
+    per launch of 64 passes into FP64 on the host.  The reference's own noise adds
+    spp / ref_spp of the image's variance to the squared RMSE; the value is corrected
+    for it."""
     import numpy as np
     from zillumgl_amd import _lib
     ctx.clear()
@@ -337,22 +340,29 @@ def rmse_vs_converged(ctx, W, H, spp, depth, ref_spp):
     img = ctx.read_frame()[..., :3].astype(np.float64) / spp
     acc = np.zeros((H, W, 3), np.float64)
     chunk = 64
+    t0 = time.time()
     for first in range(spp, spp + ref_spp, chunk):  # passes disjoint from the image's
-        ctx.clear()
-        ctx.render(_lib.INTEGRATOR_NAIVE, first, chunk, 1 + first, depth)
-        acc += ctx.read_frame()[..., :3]
+        if precision == "f64":
+            ctx.render_reference_f64(first, chunk, 1 + first, depth, rgb_sum=acc)
+        else:
+            ctx.clear()
+            ctx.render(_lib.INTEGRATOR_NAIVE, first, chunk, 1 + first, depth)
+            acc += ctx.read_frame()[..., :3]
         done = first + chunk - spp
-        if done % (chunk * 128) == 0:
-            print(f"[bench] reference passes {done}/{ref_spp}", file=sys.stderr, flush=True)
+        if done % (chunk * (4 if precision == "f64" else 128)) == 0:
+            print(f"[bench] {precision} reference passes {done}/{ref_spp} ({time.time() - t0:.0f} s)",
+                  file=sys.stderr, flush=True)
     ref = acc / ref_spp
     mse = float(((img - ref) ** 2).mean())
     mse_corrected = mse / (1.0 + spp / ref_spp)
-    return {"reference_spp": ref_spp, "rmse": round(mse_corrected ** 0.5, 6), "rmse_raw": round(mse ** 0.5, 6),
+    return {"reference_spp": ref_spp, "reference": "FP64 (zgl_render_reference_f64)" if precision == "f64"
+            else "FP32 kernel, FP64 sums", "reference_s": round(time.time() - t0, 1),
+            "rmse": round(mse_corrected ** 0.5, 6), "rmse_raw": round(mse ** 0.5, 6),
             "mean_radiance": round(float(ref.mean()), 6),
             "relative_rmse": round(mse_corrected ** 0.5 / float(ref.mean()), 6),
             "note": "same estimator and Sobol indices on the CPU oracle (frames equal per pixel to 1e-4), so "
-                    "the CPU baseline's RMSE is this one; the FP32 converged image is unbiased against the FP64 "
-                    "estimator at the same samples (tests/test_gpu_rmse.py)"}
+                    "the CPU baseline's RMSE is this one; the FP64 reference equals the oracle's FP64 mode per "
+                    "pixel (tests/test_gpu_reference64.py)"}
 
 
 def main():
@@ -368,8 +378,10 @@ def main():
     ap.add_argument("--no-bvh", action="store_true")
     ap.add_argument("--no-integrators", action="store_true")
     ap.add_argument("--bvh-only", action="store_true", help="only the MTBVH build line (profiling)")
-    ap.add_argument("--reference-spp", type=int, default=4096, metavar="N",
+    ap.add_argument("--reference-spp", type=int, default=1024, metavar="N",
                     help="RMSE against an N-spp converged image (N a multiple of 64; 0 = skip)")
+    ap.add_argument("--reference-precision", choices=["f64", "f32"], default="f64",
+                    help="the converged image's arithmetic: the FP64 reference renderer (default) or the FP32 kernel")
     ap.add_argument("--threshold", type=int, default=0, help="persistent schedule regeneration threshold (0 = default)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES (PMC calibration) and exit")
@@ -437,7 +449,7 @@ def main():
 
     rmse = None
     if args.reference_spp:
-        rmse = rmse_vs_converged(ctx, W, H, spp, depth, args.reference_spp)
+        rmse = rmse_vs_converged(ctx, W, H, spp, depth, args.reference_spp, args.reference_precision)
         print(f"[bench] RMSE vs {args.reference_spp} spp: {rmse['rmse']}", file=sys.stderr, flush=True)
 
     roofline = None

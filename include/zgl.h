@@ -198,6 +198,14 @@ zgl_status zgl_render(zgl_context* ctx, const zgl_pass_params* params);
  * (8,372,144 pixels: 3840 x 2160 fits), and any other integrator.  Every call on one GPU shares that GPU's array: calls that overlap on one
  * GPU (contexts on different streams) race; calls on one stream are ordered. */
 zgl_status zgl_render_static(zgl_context* ctx, const zgl_pass_params* params);
+/* The converged reference of the naive integrator (path_integ_naive.glsl:35-174): the
+ * same estimator with every quantity in FP64 except the FP32 scene data, literals and
+ * random numbers, one thread per pixel.  Adds each pixel's sum of the n_passes samples
+ * (NaN samples skipped) to rgb_sum, a host array of film_width x film_height x 3 doubles
+ * (RGB, row-major), which it reads and writes; returns when it is written.  For
+ * measuring the RMSE of the FP32 renderer (whose frames it does not touch); it runs at
+ * a fraction of zgl_render's rate.  Only ZGL_INTEGRATOR_NAIVE, without count_bytes. */
+zgl_status zgl_render_reference_f64(zgl_context* ctx, const zgl_pass_params* params, double* rgb_sum);
 /* Zero the frame (RGBA32F) and the light-splat film (planar R32F, 3W x H). */
 zgl_status zgl_clear_frame(zgl_context* ctx);
 /* Display transform of the frame (post_proc.glsl:1-59, called by Application::integrate,

@@ -83,6 +83,9 @@ def lib():
         L.orc_tex_lookup.argtypes = [C.POINTER(OrcScene), C.POINTER(OrcParams), C.c_int, _fp, C.c_int64, _fp]
         L.orc_fmath.argtypes = [C.c_int, _fp, _fp, _fp, C.c_int64]
         L.orc_fmath.restype = None
+        _dp = C.POINTER(C.c_double)
+        L.orc_dmath.argtypes = [C.c_int, _dp, _dp, _dp, C.c_int64]
+        L.orc_dmath.restype = None
         L.orc_bsdf_eval_f64.argtypes = [_dp, C.c_int, _dp, _dp, _dp, _dp]
         L.orc_render_replay.argtypes = [C.POINTER(OrcScene), C.POINTER(OrcParams), _fp, _fp, C.c_int, C.c_int,
                                         C.c_int, C.c_int, _fp, C.c_int]
@@ -219,6 +222,17 @@ def fmath(op, a, b=None):
     y = np.ascontiguousarray(b if b is not None else np.zeros_like(x), np.float32)
     out = np.empty_like(x)
     lib().orc_fmath(ops[op], _p(x, C.c_float), _p(y, C.c_float), _p(out, C.c_float), x.size)
+    return out
+
+
+def dmath(op, a, b=None):
+    """The FP64 routines (the oracle's double mode): op as in fmath."""
+    ops = {"sin": 0, "cos": 1, "atan2": 2, "log": 3, "exp": 4, "pow": 5}
+    x = np.ascontiguousarray(a, np.float64)
+    y = np.ascontiguousarray(b if b is not None else np.zeros_like(x), np.float64)
+    out = np.empty_like(x)
+    d = C.POINTER(C.c_double)
+    lib().orc_dmath(ops[op], x.ctypes.data_as(d), y.ctypes.data_as(d), out.ctypes.data_as(d), x.size)
     return out
 
 

@@ -20,7 +20,8 @@
 //   GL_LINEAR/GL_REPEAT filtering in FP32 with weights (1-a)(1-b), a(1-b), (1-a)b, ab
 //   summed in that order;  sin/cos/atan2/log/exp for R = float by Cody-Waite
 //   reduction + truncated Taylor series (identical steps to the GPU code; pinned
-//   against libm double in the tests).
+//   against libm double in the tests), and for R = double by the same methods with
+//   longer series (identical steps to the GPU code's FP64 instance).
 #include <algorithm>
 #include <array>
 #include <atomic>
@@ -36,7 +37,7 @@
 namespace orc {
 
 // ---------------------------------------------------------------------------
-// FP32 transcendentals (Cody-Waite + Taylor), and the FP64 versions from libm.
+// FP32 transcendentals (Cody-Waite + Taylor), and the same methods in FP64.
 // ---------------------------------------------------------------------------
 static inline float bitsToF(uint32_t b) { float f; std::memcpy(&f, &b, 4); return f; }
 static inline uint32_t fToBits(float f) { uint32_t b; std::memcpy(&b, &f, 4); return b; }
@@ -135,6 +136,152 @@ static float powF(float x, float y)
     return expF(y * logF(x));
 }
 
+// The same methods in double (the FP64 mode): pi/2 in parts of 33, 33 and 53 bits,
+// sin/cos through r^21 / r^22, atan through t^43, log's atanh series through f^23 with
+// ln 2 in parts of 32 and 53 bits, exp through r^17 (identical steps to the GPU code's
+// FP64 instance; pinned against libm double in the tests).
+static void sincosD(double x, double* s, double* c)
+{
+    const double TwoOverPi = 0.63661977236758134;
+    const double C1 = 1.5707963267341256, C2 = 6.077100506303966e-11, C3 = 2.0222662487959506e-21;
+    double kf = std::floor(x * TwoOverPi + 0.5);
+    long long k = (long long)kf;
+    double r = ((x - kf * C1) - kf * C2) - kf * C3;
+    double r2 = r * r;
+    double ps = 1.0 / 51090942171709440000.0;
+    ps = -1.0 / 121645100408832000.0 + r2 * ps;
+    ps = 1.0 / 355687428096000.0 + r2 * ps;
+    ps = -1.0 / 1307674368000.0 + r2 * ps;
+    ps = 1.0 / 6227020800.0 + r2 * ps;
+    ps = -1.0 / 39916800.0 + r2 * ps;
+    ps = 1.0 / 362880.0 + r2 * ps;
+    ps = -1.0 / 5040.0 + r2 * ps;
+    ps = 1.0 / 120.0 + r2 * ps;
+    ps = -1.0 / 6.0 + r2 * ps;
+    double pc = -1.0 / 1124000727777607680000.0;
+    pc = 1.0 / 2432902008176640000.0 + r2 * pc;
+    pc = -1.0 / 6402373705728000.0 + r2 * pc;
+    pc = 1.0 / 20922789888000.0 + r2 * pc;
+    pc = -1.0 / 87178291200.0 + r2 * pc;
+    pc = 1.0 / 479001600.0 + r2 * pc;
+    pc = -1.0 / 3628800.0 + r2 * pc;
+    pc = 1.0 / 40320.0 + r2 * pc;
+    pc = -1.0 / 720.0 + r2 * pc;
+    pc = 1.0 / 24.0 + r2 * pc;
+    double sr = r + r * r2 * ps;
+    double cr = (1.0 - 0.5 * r2) + r2 * r2 * pc;
+    switch (k & 3) {
+    case 0: *s = sr; *c = cr; break;
+    case 1: *s = cr; *c = -sr; break;
+    case 2: *s = -sr; *c = -cr; break;
+    default: *s = -cr; *c = sr; break;
+    }
+}
+static double atanUnitD(double t)
+{
+    double off = 0.0;
+    if (t > 0.41421356237309505) { t = (t - 1.0) / (t + 1.0); off = 0.78539816339744831; }
+    double t2 = t * t;
+    double p = -1.0 / 43.0;
+    p = 1.0 / 41.0 + t2 * p;
+    p = -1.0 / 39.0 + t2 * p;
+    p = 1.0 / 37.0 + t2 * p;
+    p = -1.0 / 35.0 + t2 * p;
+    p = 1.0 / 33.0 + t2 * p;
+    p = -1.0 / 31.0 + t2 * p;
+    p = 1.0 / 29.0 + t2 * p;
+    p = -1.0 / 27.0 + t2 * p;
+    p = 1.0 / 25.0 + t2 * p;
+    p = -1.0 / 23.0 + t2 * p;
+    p = 1.0 / 21.0 + t2 * p;
+    p = -1.0 / 19.0 + t2 * p;
+    p = 1.0 / 17.0 + t2 * p;
+    p = -1.0 / 15.0 + t2 * p;
+    p = 1.0 / 13.0 + t2 * p;
+    p = -1.0 / 11.0 + t2 * p;
+    p = 1.0 / 9.0 + t2 * p;
+    p = -1.0 / 7.0 + t2 * p;
+    p = 1.0 / 5.0 + t2 * p;
+    p = -1.0 / 3.0 + t2 * p;
+    return off + (t + t * t2 * p);
+}
+static double atan2D(double y, double x)
+{
+    double ax = std::fabs(x), ay = std::fabs(y);
+    double mx = (ax < ay) ? ay : ax, mn = (ay < ax) ? ay : ax;
+    if (mx == 0.0) return 0.0;
+    double a = atanUnitD(mn / mx);
+    if (ay > ax) a = 1.5707963267948966 - a;
+    if (x < 0.0) a = 3.1415926535897932 - a;
+    if (y < 0.0) a = -a;
+    return a;
+}
+static double logD(double x)
+{
+    if (!(x > 0.0)) return (x == 0.0) ? -INFINITY : NAN;
+    if (x == INFINITY) return x;
+    uint64_t b; std::memcpy(&b, &x, 8);
+    int e = (int)((b >> 52) & 0x7ff);
+    if (e == 0) { x = x * 18014398509481984.0; std::memcpy(&b, &x, 8); e = (int)((b >> 52) & 0x7ff) - 54; }
+    e -= 1023;
+    uint64_t mb = (b & 0x000fffffffffffffull) | 0x3ff0000000000000ull;
+    double m; std::memcpy(&m, &mb, 8);
+    if (m > 1.4142135623730951) { m = m * 0.5; e += 1; }
+    double f = (m - 1.0) / (m + 1.0);
+    double f2 = f * f;
+    double p = 1.0 / 23.0;
+    p = 1.0 / 21.0 + f2 * p;
+    p = 1.0 / 19.0 + f2 * p;
+    p = 1.0 / 17.0 + f2 * p;
+    p = 1.0 / 15.0 + f2 * p;
+    p = 1.0 / 13.0 + f2 * p;
+    p = 1.0 / 11.0 + f2 * p;
+    p = 1.0 / 9.0 + f2 * p;
+    p = 1.0 / 7.0 + f2 * p;
+    p = 1.0 / 5.0 + f2 * p;
+    p = 1.0 / 3.0 + f2 * p;
+    double lm = 2.0 * f + 2.0 * f * f2 * p;
+    double ef = (double)e;
+    return ef * 0.69314718036912382 + (lm + ef * 1.9082149292705877e-10);
+}
+static double expD(double x)
+{
+    if (std::isnan(x)) return x;
+    if (x > 709.78) return INFINITY;
+    if (x < -745.2) return 0.0;
+    double kf = std::floor(x * 1.4426950408889634 + 0.5);
+    int k = (int)kf;
+    double r = (x - kf * 0.69314718036912382) - kf * 1.9082149292705877e-10;
+    double p = 1.0 / 355687428096000.0;
+    p = 1.0 / 20922789888000.0 + r * p;
+    p = 1.0 / 1307674368000.0 + r * p;
+    p = 1.0 / 87178291200.0 + r * p;
+    p = 1.0 / 6227020800.0 + r * p;
+    p = 1.0 / 479001600.0 + r * p;
+    p = 1.0 / 39916800.0 + r * p;
+    p = 1.0 / 3628800.0 + r * p;
+    p = 1.0 / 362880.0 + r * p;
+    p = 1.0 / 40320.0 + r * p;
+    p = 1.0 / 5040.0 + r * p;
+    p = 1.0 / 720.0 + r * p;
+    p = 1.0 / 120.0 + r * p;
+    p = 1.0 / 24.0 + r * p;
+    p = 1.0 / 6.0 + r * p;
+    p = 1.0 / 2.0 + r * p;
+    p = 1.0 + r * p;
+    p = 1.0 + r * p;
+    int k1 = k / 2, k2 = k - k1;
+    uint64_t b1 = (uint64_t)(k1 + 1023) << 52, b2 = (uint64_t)(k2 + 1023) << 52;
+    double s1, s2; std::memcpy(&s1, &b1, 8); std::memcpy(&s2, &b2, 8);
+    return p * s1 * s2;
+}
+static double powD(double x, double y)
+{
+    if (x == 1.0 || y == 0.0) return 1.0;
+    if (x == 0.0) return (y > 0.0) ? 0.0 : INFINITY;
+    return expD(y * logD(x));
+}
+
 template <class R> struct M;
 template <> struct M<float> {
     static void sincos(float x, float* s, float* c) { sincosF(x, s, c); }
@@ -145,10 +292,10 @@ template <> struct M<float> {
     static constexpr float PiInv = 1.0f / 3.14159265358979323846f;
 };
 template <> struct M<double> {
-    static void sincos(double x, double* s, double* c) { *s = std::sin(x); *c = std::cos(x); }
-    static double atan2(double y, double x) { return (x == 0.0 && y == 0.0) ? 0.0 : std::atan2(y, x); }
-    static double log(double x) { return std::log(x); }
-    static double pow(double x, double y) { return std::pow(x, y); }
+    static void sincos(double x, double* s, double* c) { sincosD(x, s, c); }
+    static double atan2(double y, double x) { return atan2D(y, x); }
+    static double log(double x) { return logD(x); }
+    static double pow(double x, double y) { return powD(x, y); }
     static constexpr double Pi = 3.14159265358979323846;
     static constexpr double PiInv = 1.0 / 3.14159265358979323846;
 };
@@ -598,8 +745,10 @@ template <class R> struct Ctx {
             sincosF(sc.env_rotation, &rotS, &rotC);
             sincosF(-sc.env_rotation, &rotSN, &rotCN);
         } else {
-            rotS = std::sin((double)sc.env_rotation); rotC = std::cos((double)sc.env_rotation);
-            rotSN = std::sin(-(double)sc.env_rotation); rotCN = std::cos(-(double)sc.env_rotation);
+            double a, b, c, d;
+            sincosD((double)sc.env_rotation, &a, &b);
+            sincosD(-(double)sc.env_rotation, &c, &d);
+            rotS = a; rotC = b; rotSN = c; rotCN = d;
         }
     }
 
@@ -2383,6 +2532,22 @@ void orc_fmath(int op, const float* in, const float* in2, float* out, int64_t n)
         case 3: out[i] = logF(in[i]); break;
         case 4: out[i] = expF(in[i]); break;
         default: out[i] = powF(in[i], in2[i]); break;
+        }
+    }
+}
+
+// The FP64 routines, for the tests that pin them (op as orc_fmath).
+void orc_dmath(int op, const double* in, const double* in2, double* out, int64_t n)
+{
+    for (int64_t i = 0; i < n; i++) {
+        double s, c;
+        switch (op) {
+        case 0: sincosD(in[i], &s, &c); out[i] = s; break;
+        case 1: sincosD(in[i], &s, &c); out[i] = c; break;
+        case 2: out[i] = atan2D(in[i], in2[i]); break;
+        case 3: out[i] = logD(in[i]); break;
+        case 4: out[i] = expD(in[i]); break;
+        default: out[i] = powD(in[i], in2[i]); break;
         }
     }
 }

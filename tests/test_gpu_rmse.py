@@ -1,7 +1,8 @@
-"""The converged image that the headline RMSE is measured against is rendered by the
-FP32 kernel (each sample in FP32, launch sums added in FP64 on the host, as bench.py's
-rmse_vs_converged does).  Here it is compared, on crops of the 1080p headline scene,
-with the FP64 estimator at the same Sobol indices (the oracle in FP64).
+"""The FP32 renderer's samples are unbiased against the FP64 estimator.  bench.py's RMSE
+uses the FP64 reference renderer (tests/test_gpu_reference64.py) by default and the FP32
+kernel with FP64 launch sums with --reference-precision f32; this pins the second choice
+and the FP32 estimator itself: on crops of the 1080p headline scene the FP32 GPU frames
+are compared with the FP64 estimator at the same Sobol indices (the oracle in FP64).
 
 At matched samples the two differ where a rounding flips a discrete decision (which
 texel of the 5e3-radiance sun an env sample or a bounce lands on, a roulette or lobe

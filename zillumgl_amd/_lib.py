@@ -85,6 +85,7 @@ SIGNATURES = [
     ("zgl_trace_any", C.c_int32, [C.c_void_p, _fp, _fp, C.c_int64, _u8p]),
     ("zgl_render", C.c_int32, [C.c_void_p, C.POINTER(PassParams)]),
     ("zgl_render_static", C.c_int32, [C.c_void_p, C.POINTER(PassParams)]),
+    ("zgl_render_reference_f64", C.c_int32, [C.c_void_p, C.POINTER(PassParams), C.POINTER(C.c_double)]),
     ("zgl_clear_frame", C.c_int32, [C.c_void_p]),
     ("zgl_post_process", C.c_int32, [C.c_void_p, C.c_float, C.c_int32, _fp, C.c_int64, _u8p, C.c_int64]),
     ("zgl_read_frame", C.c_int32, [C.c_void_p, _fp, C.c_int64]),

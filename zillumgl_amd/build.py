@@ -47,7 +47,7 @@ def build(verbose=True):
     os.makedirs(BUILD, exist_ok=True)
     objs = []
     jobs = []
-    for src in ("zgl_kernels.hip", "zgl_persistent.hip", "zgl_adjoint.hip", "zgl_bvh.hip", "zgl_queue.hip", "zgl_wavefront.hip", "zgl_lightpath.hip", "zgl_capi.hip"):
+    for src in ("zgl_kernels.hip", "zgl_persistent.hip", "zgl_adjoint.hip", "zgl_bvh.hip", "zgl_queue.hip", "zgl_wavefront.hip", "zgl_lightpath.hip", "zgl_reference.hip", "zgl_capi.hip"):
         obj = os.path.join(BUILD, src + ".o")
         jobs.append([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)

@@ -156,6 +156,7 @@ struct zgl_context {
     int maxTexId = -1;             // largest texture id the uploaded scene's triangles use
     DevBuf naiveRes;  // per-sample values of the persistent naive tracer
     DevBuf hitLog;    // per-sample traversal results for the adjoint
+    DevBuf refSum;    // FP64 reference: per-pixel RGB sums of the passes rendered so far
     int wfTraceBlocksPerCU = 0, wfShadeBlocksPerCU = 0;
     DeviceBvh bvh;     // work memory of the device MTBVH build
     std::vector<float> hostMaterials;  // uploaded material records (16 floats each)
@@ -177,7 +178,7 @@ struct zgl_context {
                          &lightPower, &lightAlias, &lightProb, &env, &envAlias, &envProb, &noise, &sobolM,
                          &sobolTab, &frame, &film, &counters, &scratchRays, &scratchOut, &workCounter,
                          &evalMat, &evalEnv, &dFrame, &dMat, &dEnv, &bvhBounds, &gqQueues, &gqCounts, &wfWork,
-                         &naiveRes, &hitLog, &tex, &texScale, &texRaw};
+                         &naiveRes, &hitLog, &refSum, &tex, &texScale, &texRaw};
         for (DevBuf* b : all) b->release();
         bvh.release();
     }
@@ -513,6 +514,7 @@ zgl_status zgl_upload_scene(zgl_context* ctx, const zgl_scene_buffers* sb)
         s.envSum = sb->env_sum;
         sincosZ(sb->env_rotation, &s.envRotSin, &s.envRotCos);
         sincosZ(-sb->env_rotation, &s.envRotSinN, &s.envRotCosN);
+        s.envRotation = sb->env_rotation;
         s.sampler = sb->sampler ? 1 : 0;
         // evaluation point of the differentiable mode: the uploaded values
         ctx->hostMaterials.assign(sb->materials, sb->materials + (size_t)sb->n_materials * 16);
@@ -885,6 +887,26 @@ zgl_status zgl_render_static(zgl_context* ctx, const zgl_pass_params* p)
                                            ctx->numCUs * ctx->naiveBlocksPerCU, cnt, ctx->stream),
                      "naive pass");
         }
+    });
+}
+
+// The naive integrator's passes in FP64 (the converged reference), added to a host array
+// of per-pixel RGB sums.
+zgl_status zgl_render_reference_f64(zgl_context* ctx, const zgl_pass_params* p, double* rgb_sum)
+{
+    return guarded(ctx, [&] {
+        if (!rgb_sum) invalid("rgb_sum is NULL");
+        if (p && p->integrator != ZGL_INTEGRATOR_NAIVE) invalid("the FP64 reference renders the naive integrator only");
+        if (p && p->count_bytes) invalid("the FP64 reference does not count bytes");
+        DevScene s;
+        PassParams pp;
+        preparePasses(ctx, p, s, pp);
+        const size_t bytes = (size_t)ctx->filmW * ctx->filmH * 3 * sizeof(double);
+        ctx->ensure(ctx->refSum, bytes, "FP64 reference sums");
+        hipCheck(hipMemcpyAsync(ctx->refSum.p, rgb_sum, bytes, hipMemcpyHostToDevice, ctx->stream), "reference upload");
+        hipCheck(launchNaiveF64(s, pp, (double*)ctx->refSum.p, ctx->stream), "FP64 reference pass");
+        hipCheck(hipMemcpyAsync(rgb_sum, ctx->refSum.p, bytes, hipMemcpyDeviceToHost, ctx->stream), "reference download");
+        hipCheck(hipStreamSynchronize(ctx->stream), "FP64 reference");
     });
 }
 

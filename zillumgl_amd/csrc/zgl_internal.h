@@ -72,6 +72,9 @@ hipError_t launchNaiveLogged(const DevScene& s, const PassParams& p, float4* res
                              int threshold, int blocks, hipStream_t st);
 int naiveBatchBlocksPerCU();
 // maxPixels: the largest filmW x filmH of the batch (sizes the accumulate launch).
+// The naive integrator's passes in FP64, one thread per pixel (zgl_reference.hip); sum
+// holds W*H*3 per-pixel RGB sums and is added to.
+hipError_t launchNaiveF64(const DevScene& s, const PassParams& p, double* sum, hipStream_t st);
 hipError_t launchNaiveBatch(const BatchScene* items, int nScenes, int maxPixels, int* sceneCounters,
                             int blocks, hipStream_t st);
 // Differentiable naive estimator: dFrame == null renders the replay estimator into

@@ -1,4 +1,10 @@
-// zgl_math.h -- FP32 vector math with GLSL 4.50 semantics for the rendering kernels.
+// zgl_math.h -- vector math with GLSL 4.50 semantics for the rendering kernels.
+//
+// The math itself is in zgl_math_body.inc, written over a working precision `real`:
+// namespace zgl instantiates it with real = float (the rendering path, described
+// below); zgl_f64.h instantiates it again with real = double in zgl::f64 for the
+// converged FP64 reference, whose sin/cos/atan2/log/exp/pow are double routines by the
+// same methods.
 //
 // Every operation here is written so that the same source gives the same bits on
 // gfx950 and on an x86-64 host when both are compiled with -ffp-contract=off:
@@ -25,20 +31,6 @@
 
 namespace zgl {
 
-constexpr float Pi = 3.14159265358979323846f;
-constexpr float PiInv = 1.0f / 3.14159265358979323846f;  // float division, as the GLSL const
-
-ZGL_HD float fminG(float x, float y) { return (y < x) ? y : x; }
-ZGL_HD float fmaxG(float x, float y) { return (x < y) ? y : x; }
-ZGL_HD float fclampG(float x, float lo, float hi) { return fminG(fmaxG(x, lo), hi); }
-ZGL_HD float fabsG(float x) { return __builtin_fabsf(x); }
-ZGL_HD float fsqrtG(float x) { return __builtin_sqrtf(x); }
-ZGL_HD float ffloorG(float x) { return __builtin_floorf(x); }
-ZGL_HD bool fisnan(float x) { return __builtin_isnan(x); }
-ZGL_HD float square(float x) { return x * x; }
-ZGL_HD float pow5(float x) { float x2 = x * x; return x2 * x2 * x; }
-ZGL_HD float mixf(float x, float y, float a) { return x * (1.0f - a) + y * a; }
-
 ZGL_HD uint32_t fbits(float x) { return __builtin_bit_cast(uint32_t, x); }
 ZGL_HD float bitsf(uint32_t x) { return __builtin_bit_cast(float, x); }
 ZGL_HD int32_t fbitsi(float x) { return __builtin_bit_cast(int32_t, x); }
@@ -49,13 +41,13 @@ ZGL_HD float ibitsf(int32_t x) { return __builtin_bit_cast(float, x); }
 // k*C1 and k*C2 are exact for |k| < 2^15; sin and cos of r in [-pi/4, pi/4] by their
 // Taylor series through r^9 / r^10 (truncation error < 2e-9).
 // ---------------------------------------------------------------------------
-ZGL_HD void sincosZ(float x, float* s, float* c)
+ZGL_HD void sincosF32(float x, float* s, float* c)
 {
     const float TwoOverPi = 0.636619772367581343f;
     const float C1 = 1.5703125f;
     const float C2 = 4.837512969970703125e-4f;
     const float C3 = 7.54978995489188216e-8f;
-    float kf = ffloorG(x * TwoOverPi + 0.5f);
+    float kf = __builtin_floorf(x * TwoOverPi + 0.5f);
     int k = (int)kf;
     float r = ((x - kf * C1) - kf * C2) - kf * C3;
     float r2 = r * r;
@@ -70,12 +62,12 @@ ZGL_HD void sincosZ(float x, float* s, float* c)
     default: *s = -cr; *c = sr; break;
     }
 }
-ZGL_HD float sinZ(float x) { float s, c; sincosZ(x, &s, &c); return s; }
-ZGL_HD float cosZ(float x) { float s, c; sincosZ(x, &s, &c); return c; }
+ZGL_HD float sinF32(float x) { float s, c; sincosF32(x, &s, &c); return s; }
+ZGL_HD float cosF32(float x) { float s, c; sincosF32(x, &s, &c); return c; }
 
 // atan on [0, 1]: reduce t > tan(pi/8) with atan(t) = pi/4 + atan((t-1)/(t+1)), then
 // the Taylor series through t^19 on |t| <= 0.4143 (truncation error < 1e-9).
-ZGL_HD float atanUnitZ(float t)
+ZGL_HD float atanUnitF32(float t)
 {
     float off = 0.0f;
     if (t > 0.41421356237309505f) { t = (t - 1.0f) / (t + 1.0f); off = 0.785398163397448310f; }
@@ -93,12 +85,12 @@ ZGL_HD float atanUnitZ(float t)
 }
 
 // GLSL atan(y, x); (0, 0) returns 0.
-ZGL_HD float atan2Z(float y, float x)
+ZGL_HD float atan2F32(float y, float x)
 {
-    float ax = fabsG(x), ay = fabsG(y);
-    float mx = fmaxG(ax, ay), mn = fminG(ax, ay);
+    float ax = __builtin_fabsf(x), ay = __builtin_fabsf(y);
+    float mx = (ax < ay) ? ay : ax, mn = (ay < ax) ? ay : ax;
     if (mx == 0.0f) return 0.0f;
-    float a = atanUnitZ(mn / mx);
+    float a = atanUnitF32(mn / mx);
     if (ay > ax) a = 1.57079632679489662f - a;
     if (x < 0.0f) a = 3.14159265358979324f - a;
     if (y < 0.0f) a = -a;
@@ -107,7 +99,7 @@ ZGL_HD float atan2Z(float y, float x)
 
 // Natural log for finite x > 0: x = m * 2^e with m in [sqrt(1/2), sqrt(2)),
 // log(m) = 2 atanh(f), f = (m-1)/(m+1), series through f^11.
-ZGL_HD float logZ(float x)
+ZGL_HD float logF32(float x)
 {
     if (!(x > 0.0f)) return (x == 0.0f) ? -__builtin_inff() : __builtin_nanf("");
     if (x == __builtin_inff()) return x;
@@ -134,12 +126,12 @@ ZGL_HD float logZ(float x)
 }
 
 // exp: x = k ln2 + r, |r| <= ln2/2, Taylor through r^8, then scale by 2^k.
-ZGL_HD float expZ(float x)
+ZGL_HD float expF32(float x)
 {
-    if (fisnan(x)) return x;
+    if (__builtin_isnan(x)) return x;
     if (x > 88.7228f) return __builtin_inff();
     if (x < -103.97f) return 0.0f;
-    float kf = ffloorG(x * 1.44269504088896341f + 0.5f);
+    float kf = __builtin_floorf(x * 1.44269504088896341f + 0.5f);
     int k = (int)kf;
     float r = (x - kf * 0.693145751953125f) - kf * 1.42860682030941723e-6f;
     float p = 1.0f / 40320.0f;
@@ -157,180 +149,181 @@ ZGL_HD float expZ(float x)
 }
 
 // pow for x > 0 (the only use is the GTR1 sample with alpha in (0, 1]).
-ZGL_HD float powZ(float x, float y)
+ZGL_HD float powF32(float x, float y)
 {
     if (x == 1.0f || y == 0.0f) return 1.0f;
     if (x == 0.0f) return (y > 0.0f) ? 0.0f : __builtin_inff();
-    return expZ(y * logZ(x));
+    return expF32(y * logF32(x));
 }
 
 // ---------------------------------------------------------------------------
-// Small vectors.
+// FP64 transcendentals of the double instance, by the same methods as the FP32 ones
+// with double constants and longer series, so that the GPU and a host build give the
+// same bits (the libraries' double routines differ in the last ulp, and sampling steps
+// such as sqrt(1 - x^2) near x = 1 amplify that).  Accuracy: a few ulp of double.
+// sin/cos: pi/2 split into parts of 33, 33 and 53 bits (k*C1 and k*C2 exact for
+// |k| < 2^20), Taylor series through r^21 / r^22 on |r| <= pi/4 (truncation < 1e-24).
 // ---------------------------------------------------------------------------
-struct vec2 { float x, y; };
-struct vec3 { float x, y, z; };
-struct vec4 { float x, y, z, w; };
-
-ZGL_HD vec2 v2(float x, float y) { return vec2{x, y}; }
-ZGL_HD vec3 v3(float x, float y, float z) { return vec3{x, y, z}; }
-ZGL_HD vec3 v3(float s) { return vec3{s, s, s}; }
-ZGL_HD vec4 v4(float x, float y, float z, float w) { return vec4{x, y, z, w}; }
-
-ZGL_HD vec3 operator+(vec3 a, vec3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
-ZGL_HD vec3 operator-(vec3 a, vec3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
-ZGL_HD vec3 operator*(vec3 a, vec3 b) { return v3(a.x * b.x, a.y * b.y, a.z * b.z); }
-ZGL_HD vec3 operator/(vec3 a, vec3 b) { return v3(a.x / b.x, a.y / b.y, a.z / b.z); }
-ZGL_HD vec3 operator*(vec3 a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
-ZGL_HD vec3 operator*(float s, vec3 a) { return v3(s * a.x, s * a.y, s * a.z); }
-ZGL_HD vec3 operator/(vec3 a, float s) { return v3(a.x / s, a.y / s, a.z / s); }
-ZGL_HD vec3 operator+(vec3 a, float s) { return v3(a.x + s, a.y + s, a.z + s); }
-ZGL_HD vec3 operator-(vec3 a, float s) { return v3(a.x - s, a.y - s, a.z - s); }
-ZGL_HD vec3 operator-(vec3 a) { return v3(-a.x, -a.y, -a.z); }
-// Per-component select.  A `c ? a : b` on the struct itself can be lowered to a pick
-// between two stack copies (scratch memory); three float selects stay in registers.
-ZGL_HD vec3 select3(bool c, vec3 a, vec3 b)
+ZGL_HD void sincosF64(double x, double* s, double* c)
 {
-    return v3(c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z);
-}
-ZGL_HD vec3& operator+=(vec3& a, vec3 b) { a = a + b; return a; }
-ZGL_HD vec3& operator*=(vec3& a, vec3 b) { a = a * b; return a; }
-ZGL_HD vec3& operator*=(vec3& a, float s) { a = a * s; return a; }
-ZGL_HD vec3& operator/=(vec3& a, float s) { a = a / s; return a; }
-
-ZGL_HD vec2 operator+(vec2 a, vec2 b) { return v2(a.x + b.x, a.y + b.y); }
-ZGL_HD vec2 operator-(vec2 a, vec2 b) { return v2(a.x - b.x, a.y - b.y); }
-ZGL_HD vec2 operator*(vec2 a, vec2 b) { return v2(a.x * b.x, a.y * b.y); }
-ZGL_HD vec2 operator/(vec2 a, vec2 b) { return v2(a.x / b.x, a.y / b.y); }
-ZGL_HD vec2 operator*(vec2 a, float s) { return v2(a.x * s, a.y * s); }
-ZGL_HD vec2 operator-(vec2 a, float s) { return v2(a.x - s, a.y - s); }
-
-ZGL_HD float dot(vec3 a, vec3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-ZGL_HD float dot(vec2 a, vec2 b) { return a.x * b.x + a.y * b.y; }
-ZGL_HD vec3 cross(vec3 a, vec3 b)
-{
-    return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
-}
-ZGL_HD float length(vec3 a) { return fsqrtG(dot(a, a)); }
-ZGL_HD float length(vec2 a) { return fsqrtG(dot(a, a)); }
-ZGL_HD float distance(vec3 a, vec3 b) { return length(a - b); }
-// normalize(v) = v * (1 / |v|)
-ZGL_HD vec3 normalize(vec3 a) { float inv = 1.0f / fsqrtG(dot(a, a)); return a * inv; }
-ZGL_HD vec3 vmin(vec3 a, vec3 b) { return v3(fminG(a.x, b.x), fminG(a.y, b.y), fminG(a.z, b.z)); }
-ZGL_HD vec3 vmax(vec3 a, vec3 b) { return v3(fmaxG(a.x, b.x), fmaxG(a.y, b.y), fmaxG(a.z, b.z)); }
-ZGL_HD vec3 vabs(vec3 a) { return v3(fabsG(a.x), fabsG(a.y), fabsG(a.z)); }
-ZGL_HD vec3 mix(vec3 x, vec3 y, float a) { return x * (1.0f - a) + y * a; }
-// GLSL reflect(I, N) = I - 2 dot(N, I) N
-ZGL_HD vec3 reflect(vec3 i, vec3 n) { return i - n * (2.0f * dot(n, i)); }
-ZGL_HD float satDot(vec3 a, vec3 b) { return fmaxG(dot(a, b), 0.0f); }
-ZGL_HD float absDot(vec3 a, vec3 b) { return fabsG(dot(a, b)); }
-ZGL_HD float distSquare(vec3 x, vec3 y) { return dot(x - y, x - y); }
-ZGL_HD float maxComponent(vec3 v) { return fmaxG(v.x, fmaxG(v.y, v.z)); }
-// Rec. 601 weights, used for light selection and path logic (math.glsl:193-196)
-ZGL_HD float luminance(vec3 c) { return dot(c, v3(0.299f, 0.587f, 0.114f)); }
-ZGL_HD bool isBlack(vec3 c) { return luminance(c) < 1e-5f; }
-ZGL_HD bool hasNan(vec3 c) { return fisnan(c.x) || fisnan(c.y) || fisnan(c.z); }
-ZGL_HD bool sameHemisphere(vec3 n, vec3 a, vec3 b) { return dot(n, a) * dot(n, b) > 0.0f; }
-
-// Column-major 3x3 matrix, M * v = c0 v.x + c1 v.y + c2 v.z.
-struct mat3 { vec3 c0, c1, c2; };
-ZGL_HD vec3 mul(const mat3& m, vec3 v) { return (m.c0 * v.x + m.c1 * v.y) + m.c2 * v.z; }
-// General inverse by cofactors over the determinant (the GLSL builtin inverse()).
-ZGL_HD mat3 inverse(const mat3& m)
-{
-    float a00 = m.c0.x, a01 = m.c0.y, a02 = m.c0.z;
-    float a10 = m.c1.x, a11 = m.c1.y, a12 = m.c1.z;
-    float a20 = m.c2.x, a21 = m.c2.y, a22 = m.c2.z;
-    float b01 = a22 * a11 - a12 * a21;
-    float b11 = -a22 * a10 + a12 * a20;
-    float b21 = a21 * a10 - a11 * a20;
-    float det = a00 * b01 + a01 * b11 + a02 * b21;
-    float id = 1.0f / det;
-    mat3 r;
-    r.c0 = v3(b01 * id, (-a22 * a01 + a02 * a21) * id, (a12 * a01 - a02 * a11) * id);
-    r.c1 = v3(b11 * id, (a22 * a00 - a02 * a20) * id, (-a12 * a00 + a02 * a10) * id);
-    r.c2 = v3(b21 * id, (-a21 * a00 + a01 * a20) * id, (a11 * a00 - a01 * a10) * id);
-    return r;
-}
-
-// math.glsl:73-89
-ZGL_HD mat3 tbnMatrix(vec3 n)
-{
-    vec3 t = (fabsG(n.z) > 0.999f) ? v3(0.0f, 1.0f, 0.0f) : v3(0.0f, 0.0f, 1.0f);
-    vec3 b = normalize(cross(n, t));
-    t = cross(b, n);
-    return mat3{t, b, n};
-}
-ZGL_HD vec3 normalToWorld(vec3 n, vec3 v) { return normalize(mul(tbnMatrix(n), v)); }
-
-// math.glsl:25-41 (Shirley-Chiu concentric map)
-ZGL_HD vec2 toConcentricDisk(vec2 v)
-{
-    if (v.x == 0.0f && v.y == 0.0f) return v2(0.0f, 0.0f);
-    v = v * 2.0f - 1.0f;
-    float phi, r;
-    if (v.x * v.x > v.y * v.y) {
-        r = v.x;
-        phi = Pi * v.y / v.x * 0.25f;
-    } else {
-        r = v.y;
-        phi = Pi * 0.5f - Pi * v.x / v.y * 0.25f;
+    const double TwoOverPi = 0.63661977236758134;
+    const double C1 = 1.5707963267341256;
+    const double C2 = 6.077100506303966e-11;
+    const double C3 = 2.0222662487959506e-21;
+    const double kf = __builtin_floor(x * TwoOverPi + 0.5);
+    const long long k = (long long)kf;
+    const double r = ((x - kf * C1) - kf * C2) - kf * C3;
+    const double r2 = r * r;
+    double ps = 1.0 / 51090942171709440000.0;  // 1/21!
+    ps = -1.0 / 121645100408832000.0 + r2 * ps;
+    ps = 1.0 / 355687428096000.0 + r2 * ps;
+    ps = -1.0 / 1307674368000.0 + r2 * ps;
+    ps = 1.0 / 6227020800.0 + r2 * ps;
+    ps = -1.0 / 39916800.0 + r2 * ps;
+    ps = 1.0 / 362880.0 + r2 * ps;
+    ps = -1.0 / 5040.0 + r2 * ps;
+    ps = 1.0 / 120.0 + r2 * ps;
+    ps = -1.0 / 6.0 + r2 * ps;
+    double pc = -1.0 / 1124000727777607680000.0;  // -1/22!
+    pc = 1.0 / 2432902008176640000.0 + r2 * pc;
+    pc = -1.0 / 6402373705728000.0 + r2 * pc;
+    pc = 1.0 / 20922789888000.0 + r2 * pc;
+    pc = -1.0 / 87178291200.0 + r2 * pc;
+    pc = 1.0 / 479001600.0 + r2 * pc;
+    pc = -1.0 / 3628800.0 + r2 * pc;
+    pc = 1.0 / 40320.0 + r2 * pc;
+    pc = -1.0 / 720.0 + r2 * pc;
+    pc = 1.0 / 24.0 + r2 * pc;
+    const double sr = r + r * r2 * ps;
+    const double cr = (1.0 - 0.5 * r2) + r2 * r2 * pc;
+    switch (k & 3) {
+    case 0: *s = sr; *c = cr; break;
+    case 1: *s = cr; *c = -sr; break;
+    case 2: *s = -sr; *c = -cr; break;
+    default: *s = -cr; *c = sr; break;
     }
-    float s, c;
-    sincosZ(phi, &s, &c);
-    return v2(r * c, r * s);
 }
 
-// math.glsl:99-105
-ZGL_HD vec4 sampleCosineWeighted(vec3 n, vec2 u)
+// atan on [0, 1]: t > tan(pi/8) reduced by atan(t) = pi/4 + atan((t-1)/(t+1)), then
+// the series through t^43 on |t| <= tan(pi/8) (truncation < 1e-17).
+ZGL_HD double atanUnitF64(double t)
 {
-    vec2 uv = toConcentricDisk(u);
-    float z = fsqrtG(1.0f - dot(uv, uv));
-    vec3 v = normalToWorld(n, v3(uv.x, uv.y, z));
-    return v4(v.x, v.y, v.z, PiInv * z);
+    double off = 0.0;
+    if (t > 0.41421356237309505) { t = (t - 1.0) / (t + 1.0); off = 0.78539816339744831; }
+    const double t2 = t * t;
+    double p = -1.0 / 43.0;
+    for (int n = 41; n >= 3; n -= 2) p = ((n & 2) ? -1.0 : 1.0) / (double)n + t2 * p;
+    return off + (t + t * t2 * p);
 }
 
-// math.glsl:58-71 (z-up equirect)
-ZGL_HD vec2 sphereToPlane(vec3 uv)
+// GLSL atan(y, x); (0, 0) returns 0.
+ZGL_HD double atan2F64(double y, double x)
 {
-    float theta = atan2Z(uv.y, uv.x);
-    if (theta < 0.0f) theta += Pi * 2.0f;
-    float phi = atan2Z(length(v2(uv.x, uv.y)), uv.z);
-    return v2(theta * PiInv * 0.5f, phi * PiInv);
-}
-ZGL_HD vec3 planeToSphere(vec2 uv)
-{
-    float theta = uv.x * Pi * 2.0f;
-    float phi = uv.y * Pi;
-    float st, ct, sp, cp;
-    sincosZ(theta, &st, &ct);
-    sincosZ(phi, &sp, &cp);
-    return v3(ct * sp, st * sp, cp);
-}
-// rotateZ with precomputed cos/sin of the angle (math.glsl:180-185)
-ZGL_HD vec3 rotateZcs(vec3 v, float cost, float sint)
-{
-    return v3(v.x * cost - v.y * sint, v.x * sint + v.y * cost, v.z);
+    const double ax = __builtin_fabs(x), ay = __builtin_fabs(y);
+    const double mx = (ax < ay) ? ay : ax, mn = (ay < ax) ? ay : ax;
+    if (mx == 0.0) return 0.0;
+    double a = atanUnitF64(mn / mx);
+    if (ay > ax) a = 1.5707963267948966 - a;
+    if (x < 0.0) a = 3.1415926535897932 - a;
+    if (y < 0.0) a = -a;
+    return a;
 }
 
-// math.glsl:112-131; table index for the MTBVH traversal of direction dir
-ZGL_HD int cubemapFace(vec3 dir)
+// Natural log for finite x > 0: x = m * 2^e, m in [sqrt(1/2), sqrt(2)), log(m) =
+// 2 atanh(f), f = (m-1)/(m+1), series through f^23; ln 2 in parts of 32 and 53 bits.
+ZGL_HD double logF64(double x)
 {
-    vec3 v = vabs(dir);
-    int maxDim = (v.x > v.y) ? (v.x > v.z ? 0 : 2) : (v.y > v.z ? 1 : 2);
-    if (maxDim == 0) return dir.x > 0.0f ? 0 : 1;
-    if (maxDim == 1) return dir.y > 0.0f ? 2 : 3;
-    return dir.z > 0.0f ? 4 : 5;
+    if (!(x > 0.0)) return (x == 0.0) ? -__builtin_inf() : __builtin_nan("");
+    if (x == __builtin_inf()) return x;
+    uint64_t b = __builtin_bit_cast(uint64_t, x);
+    int e = (int)((b >> 52) & 0x7ff);
+    if (e == 0) {  // subnormal: scale up by 2^54
+        x = x * 18014398509481984.0;
+        b = __builtin_bit_cast(uint64_t, x);
+        e = (int)((b >> 52) & 0x7ff) - 54;
+    }
+    e -= 1023;
+    double m = __builtin_bit_cast(double, (b & 0x000fffffffffffffull) | 0x3ff0000000000000ull);
+    if (m > 1.4142135623730951) { m = m * 0.5; e += 1; }
+    const double f = (m - 1.0) / (m + 1.0);
+    const double f2 = f * f;
+    double p = 1.0 / 23.0;
+    for (int n = 21; n >= 3; n -= 2) p = 1.0 / (double)n + f2 * p;
+    const double lm = 2.0 * f + 2.0 * f * f2 * p;
+    const double ef = (double)e;
+    return ef * 0.69314718036912382 + (lm + ef * 1.9082149292705877e-10);
 }
 
-// math.glsl:139-145
-ZGL_HD vec3 sampleTriangleUniform(vec3 va, vec3 vb, vec3 vc, vec2 uv)
+// exp: x = k ln2 + r, |r| <= ln2/2, Taylor through r^17, then scale by 2^k.
+ZGL_HD double expF64(double x)
 {
-    float r = fsqrtG(uv.y);
-    float u = 1.0f - r;
-    float v = uv.x * r;
-    return (va * (1.0f - u - v) + vb * u) + vc * v;
+    if (__builtin_isnan(x)) return x;
+    if (x > 709.78) return __builtin_inf();
+    if (x < -745.2) return 0.0;
+    const double kf = __builtin_floor(x * 1.4426950408889634 + 0.5);
+    const int k = (int)kf;
+    const double r = (x - kf * 0.69314718036912382) - kf * 1.9082149292705877e-10;
+    double p = 1.0 / 355687428096000.0;  // 1/17!
+    p = 1.0 / 20922789888000.0 + r * p;
+    p = 1.0 / 1307674368000.0 + r * p;
+    p = 1.0 / 87178291200.0 + r * p;
+    p = 1.0 / 6227020800.0 + r * p;
+    p = 1.0 / 479001600.0 + r * p;
+    p = 1.0 / 39916800.0 + r * p;
+    p = 1.0 / 3628800.0 + r * p;
+    p = 1.0 / 362880.0 + r * p;
+    p = 1.0 / 40320.0 + r * p;
+    p = 1.0 / 5040.0 + r * p;
+    p = 1.0 / 720.0 + r * p;
+    p = 1.0 / 120.0 + r * p;
+    p = 1.0 / 24.0 + r * p;
+    p = 1.0 / 6.0 + r * p;
+    p = 1.0 / 2.0 + r * p;
+    p = 1.0 + r * p;
+    p = 1.0 + r * p;
+    // 2^k in two steps so that k down to -1074 stays representable
+    const int k1 = k / 2, k2 = k - k1;
+    return p * __builtin_bit_cast(double, (uint64_t)(k1 + 1023) << 52) *
+           __builtin_bit_cast(double, (uint64_t)(k2 + 1023) << 52);
 }
-ZGL_HD float triangleArea(vec3 va, vec3 vb, vec3 vc) { return 0.5f * length(cross(vc - va, vb - va)); }
+
+// pow for x > 0 (the only use is the GTR1 sample with alpha in (0, 1]).
+ZGL_HD double powF64(double x, double y)
+{
+    if (x == 1.0 || y == 0.0) return 1.0;
+    if (x == 0.0) return (y > 0.0) ? 0.0 : __builtin_inf();
+    return expF64(y * logF64(x));
+}
+
+// Overloads the working-precision math of zgl_math_body.inc dispatches on.
+ZGL_HD float absImpl(float x) { return __builtin_fabsf(x); }
+ZGL_HD double absImpl(double x) { return __builtin_fabs(x); }
+ZGL_HD float sqrtImpl(float x) { return __builtin_sqrtf(x); }
+ZGL_HD double sqrtImpl(double x) { return __builtin_sqrt(x); }
+ZGL_HD float floorImpl(float x) { return __builtin_floorf(x); }
+ZGL_HD double floorImpl(double x) { return __builtin_floor(x); }
+// IEEE minNum / maxNum (one instruction each), for operands known to hold no NaN
+ZGL_HD float ieeeMin(float x, float y) { return __builtin_fminf(x, y); }
+ZGL_HD double ieeeMin(double x, double y) { return __builtin_fmin(x, y); }
+ZGL_HD float ieeeMax(float x, float y) { return __builtin_fmaxf(x, y); }
+ZGL_HD double ieeeMax(double x, double y) { return __builtin_fmax(x, y); }
+ZGL_HD void sincosImpl(float x, float* s, float* c) { sincosF32(x, s, c); }
+ZGL_HD void sincosImpl(double x, double* s, double* c) { sincosF64(x, s, c); }
+ZGL_HD float atan2Impl(float y, float x) { return atan2F32(y, x); }
+ZGL_HD double atan2Impl(double y, double x) { return atan2F64(y, x); }
+ZGL_HD float logImpl(float x) { return logF32(x); }
+ZGL_HD double logImpl(double x) { return logF64(x); }
+ZGL_HD float expImpl(float x) { return expF32(x); }
+ZGL_HD double expImpl(double x) { return expF64(x); }
+ZGL_HD float powImpl(float x, float y) { return powF32(x, y); }
+ZGL_HD double powImpl(double x, double y) { return powF64(x, y); }
+
+// Small vectors and the 3x3 matrix, over the element type.
+template <class T> struct TVec2 { T x, y; };
+template <class T> struct TVec3 { T x, y, z; };
+template <class T> struct TVec4 { T x, y, z, w; };
+// column-major: M * v = c0 v.x + c1 v.y + c2 v.z
+template <class T> struct TMat3 { TVec3<T> c0, c1, c2; };
 
 // random.glsl:3-13 -- 32-bit integer hash
 ZGL_HD uint32_t hashU(uint32_t seed)
@@ -343,5 +336,14 @@ ZGL_HD uint32_t hashU(uint32_t seed)
     return seed;
 }
 ZGL_HD float u32ToUnit(uint32_t r) { return (float)r * (1.0f / 4294967296.0f); }
+
+// The rendering path's instance of the vector math, in its own namespace so that the
+// double instance (zgl::f64) never finds these functions by argument-dependent lookup on
+// the shared types; the using-directive makes them zgl's for the kernels.
+namespace f32 {
+typedef float real;
+#include "zgl_math_body.inc"
+}  // namespace f32
+using namespace f32;
 
 }  // namespace zgl

@@ -108,6 +108,21 @@ class Context:
                             loops_per_pass, int(count_bytes), regen_threshold)
         self._check(_lib.lib().zgl_render(self.h, C.byref(p)))
 
+    def render_reference_f64(self, first_pass, n_passes, free_counter, max_depth, russian_roulette=False,
+                             sample_light=True, light_env_uniform=False, light_portion=0.5, rgb_sum=None):
+        """The naive integrator's passes in FP64 (zgl_render_reference_f64): returns
+        rgb_sum (H, W, 3 float64, a new zero array if None) with each pixel's sum of the
+        n_passes samples added."""
+        if rgb_sum is None:
+            rgb_sum = np.zeros((self.height, self.width, 3), np.float64)
+        assert rgb_sum.dtype == np.float64 and rgb_sum.shape == (self.height, self.width, 3)
+        assert rgb_sum.flags.c_contiguous
+        p = _lib.PassParams(_lib.INTEGRATOR_NAIVE, first_pass, n_passes, free_counter, max_depth,
+                            int(russian_roulette), int(sample_light), int(light_env_uniform), light_portion, 32, 1,
+                            0, 0)
+        self._check(_lib.lib().zgl_render_reference_f64(self.h, C.byref(p), ptr(rgb_sum, C.c_double)))
+        return rgb_sum
+
     # ------------------------------------------------------------ differentiable mode
     def _params(self, first_pass, n_passes, free_counter, max_depth, russian_roulette, sample_light,
                 light_env_uniform, light_portion):
