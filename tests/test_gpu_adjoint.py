@@ -4,18 +4,19 @@
   * at perturbed parameters it equals the oracle's FP32 replay (same operations), per
     pixel within 1e-4 relative;
   * the adjoint equals the oracle's FP64 adjoint of the FP32 estimator per gradient
-    component (components above 1e-3 of the largest): environment texels within 1e-3
-    relative, material parameters within 5e-3.  The oracle's BSDF derivatives are FP64
-    central differences, the kernel's are FP32 dual numbers; they agree to ~1e-5
-    except for the roughness of glossy lobes (raw roughness near the 0.05 floor of the
-    scene generator, alpha^2 ~ 1e-5), where (alpha^2 - 1) cos^2 + 1 loses 1e-3 of its
-    value to FP32 rounding;
+    component (components above 1e-3 of the largest) within 1e-4 relative, environment
+    texels and material parameters alike.  The oracle's BSDF derivatives are FP64
+    central differences; the kernel's are dual numbers, with FP64 tangents in the
+    specular lobe, where (alpha^2 - 1) cos^2 + 1 at the scene generator's roughness
+    floor (alpha^2 ~ 1e-5) would lose 1e-3 of a tangent to FP32 rounding.  Measured:
+    2.3e-5 at most;
   * through the torch.autograd.Function, the gradient of an L2 image loss against a
     target rendered at parameters perturbed by N(0, 0.1) (clamped to [0, 1]) agrees
-    with fourth-order central finite differences on 64 random parameters (material
-    steps h = 2^-10 ~ 1e-3; env steps larger, the loss being quadratic in the texels)
-    within 1e-3 relative (roughness of glossy lobes, where FP32 cannot resolve such a
-    step, is left to the FP64 oracle comparison).
+    with fourth-order central finite differences on 32 random material parameters and
+    32 random environment texels (material steps h = 2^-10 ~ 1e-3; env steps larger,
+    the loss being quadratic in the texels) within 1e-3 relative (the roughness of
+    glossy lobes, where the FP32 estimator cannot resolve such a step, is left to the
+    FP64 oracle comparison).
 """
 import numpy as np
 import pytest
@@ -113,7 +114,7 @@ def test_adjoint_matches_oracle_fp64(ctx, c2s):
     ee, ne = _close(ge, re, 1e-3)
     print(f"[gpu] adjoint: material max rel err {em:.3g} over {nm}, env {ee:.3g} over {ne}", flush=True)
     assert nm >= 20 and ne >= 1000
-    assert em <= 5e-3 and ee <= 1e-3
+    assert em <= 1e-4 and ee <= 1e-4
 
 
 def test_adjoint_offset_passes_and_split_launches(ctx, c2s):
@@ -133,7 +134,7 @@ def test_adjoint_offset_passes_and_split_launches(ctx, c2s):
     ee, ne = _close(ge, re, 1e-3)
     print(f"[gpu] adjoint offset: material max rel err {em:.3g} over {nm}, env {ee:.3g} over {ne}", flush=True)
     assert nm >= 20 and ne >= 1000
-    assert em <= 5e-3 and ee <= 1e-3
+    assert em <= 1e-4 and ee <= 1e-4
     am, ae = ctx.adjoint(d_frame, p1.shape[0], e1.shape, 5, 3, 9, DEPTH, russian_roulette=True)
     bm, be = ctx.adjoint(d_frame, p1.shape[0], e1.shape, 8, 3, 12, DEPTH, russian_roulette=True)
     for whole, parts in ((gm, am.astype(np.float64) + bm), (ge, ae.astype(np.float64) + be)):
@@ -156,7 +157,7 @@ def test_adjoint_deep_paths_match_oracle_fp64(ctx, c2s):
     ee, ne = _close(ge, re, 1e-3)
     print(f"[gpu] adjoint depth 10: material max rel err {em:.3g} over {nm}, env {ee:.3g} over {ne}", flush=True)
     assert nm >= 20 and ne >= 1000
-    assert em <= 5e-3 and ee <= 1e-3
+    assert em <= 1e-4 and ee <= 1e-4
 
 
 def test_autograd_l2_gradient_matches_central_differences(ctx, c2s):
@@ -203,9 +204,13 @@ def test_autograd_l2_gradient_matches_central_differences(ctx, c2s):
     cand_m = [i for i in range(gm.size)
               if abs(gm.ravel()[i]) >= 1e-3 * np.abs(gm).max() and (i % 5 != 3 or resolvable[i // 5])]
     cand_e = list(np.nonzero(np.abs(ge.ravel()) >= 1e-3 * np.abs(ge).max())[0])
+    # half of the draws from the material parameters: drawn with the env texels, whose
+    # candidates outnumber them a thousandfold, they would almost never be checked
     rng = np.random.default_rng(9)
-    picks = rng.choice(len(cand_m) + len(cand_e), 64, replace=False)
-    params = [("m", cand_m[k]) if k < len(cand_m) else ("e", cand_e[k - len(cand_m)]) for k in picks]
+    n_m = min(32, len(cand_m))
+    assert n_m >= 16
+    params = [("m", cand_m[k]) for k in rng.choice(len(cand_m), n_m, replace=False)]
+    params += [("e", cand_e[k]) for k in rng.choice(len(cand_e), 64 - n_m, replace=False)]
     worst = 0.0
     for kind, i in params:
         f4 = fd(kind, i)
@@ -213,6 +218,5 @@ def test_autograd_l2_gradient_matches_central_differences(ctx, c2s):
         rel = abs(g - f4) / max(abs(f4), 1e-30)
         worst = max(worst, rel)
         assert rel <= 1e-3, (kind, i, g, f4, rel)
-    n_m = sum(k == "m" for k, _ in params)
     print(f"[gpu] adjoint: 64 central differences ({n_m} material, {64 - n_m} env), worst rel err {worst:.3g}",
           flush=True)

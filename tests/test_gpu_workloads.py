@@ -8,11 +8,11 @@ as per-scene zgl_render does, and scene 34's frame equals the oracle's on bands 
 Differentiable workload: the 201,588-triangle Cornell box with 32 Disney materials and a
 256 x 128 environment at 512^2, depth 6: the adjoint against the oracle's FP64 adjoint of
 the FP32 estimator over the whole image, and the gradient of the L2 loss at 32 spp against
-fourth-order central differences on 64 random parameters.
+fourth-order central differences on 32 random material parameters and 32 random
+environment texels.
 
 Tolerances: frames bit-exact; adjoint components above 1e-3 of the largest of their kind
-within 1e-3 relative (material roughness of glossy lobes that FP32 cannot resolve, as in
-test_gpu_adjoint.py, within 5e-3); central differences within 1e-3 relative."""
+within 1e-4 relative; central differences within 1e-3 relative."""
 import numpy as np
 import pytest
 import torch
@@ -133,9 +133,8 @@ def test_differentiable_workload_adjoint_matches_oracle_fp64(ctx5, c5):
           f"(unresolvable roughness {err_m[sig_m & loose].max() if (sig_m & loose).any() else 0:.3g}), "
           f"env {err_e[sig_e].max():.3g} over {sig_e.sum()}", flush=True)
     assert sig_m.sum() >= 100 and sig_e.sum() >= 5000
-    assert err_m[sig_m & ~loose].max() <= 1e-3
-    assert not (sig_m & loose).any() or err_m[sig_m & loose].max() <= 5e-3
-    assert err_e[sig_e].max() <= 1e-3
+    assert err_m[sig_m].max() <= 1e-4
+    assert err_e[sig_e].max() <= 1e-4
 
 
 def test_differentiable_workload_gradient_matches_central_differences(ctx5, c5):
@@ -169,9 +168,13 @@ def test_differentiable_workload_gradient_matches_central_differences(ctx5, c5):
     cand_m = [i for i in range(gm.size)
               if abs(gm.ravel()[i]) >= 1e-3 * np.abs(gm).max() and (i % 5 != 3 or resolvable[i // 5])]
     cand_e = list(np.nonzero(np.abs(ge.ravel()) >= 1e-3 * np.abs(ge).max())[0])
+    # half of the draws from the material parameters (they would almost never be drawn
+    # with the far more numerous env texels)
     rng = np.random.default_rng(24)
-    picks = rng.choice(len(cand_m) + len(cand_e), 64, replace=False)
-    params = [("m", cand_m[k]) if k < len(cand_m) else ("e", cand_e[k - len(cand_m)]) for k in picks]
+    n_m = min(32, len(cand_m))
+    assert n_m >= 16
+    params = [("m", cand_m[k]) for k in rng.choice(len(cand_m), n_m, replace=False)]
+    params += [("e", cand_e[k]) for k in rng.choice(len(cand_e), 64 - n_m, replace=False)]
     worst = 0.0
     for n, (kind, i) in enumerate(params):
         f4 = fd(kind, i)
@@ -179,5 +182,6 @@ def test_differentiable_workload_gradient_matches_central_differences(ctx5, c5):
         rel = abs(g - f4) / max(abs(f4), 1e-30)
         worst = max(worst, rel)
         if n % 16 == 15:
-            print(f"[gpu] differentiable workload: {n + 1} differences, worst rel err {worst:.3g}", flush=True)
+            print(f"[gpu] differentiable workload: {n + 1} differences ({kind}), worst rel err {worst:.3g}",
+                  flush=True)
         assert rel <= 1e-3, (kind, i, g, f4, rel)
