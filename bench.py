@@ -360,7 +360,7 @@ def rmse_vs_converged(ctx, W, H, spp, depth, ref_spp, precision="f64"):
             "rmse": round(mse_corrected ** 0.5, 6), "rmse_raw": round(mse ** 0.5, 6),
             "mean_radiance": round(float(ref.mean()), 6),
             "relative_rmse": round(mse_corrected ** 0.5 / float(ref.mean()), 6),
-            "note": "same estimator and Sobol indices on the CPU oracle (frames equal per pixel to 1e-4), so "
+            "note": "same estimator and Sobol indices on the CPU oracle (frames equal bit for bit), so "
                     "the CPU baseline's RMSE is this one; the FP64 reference equals the oracle's FP64 mode per "
                     "pixel (tests/test_gpu_reference64.py)"}
 
@@ -520,8 +520,8 @@ def main():
         "dtype": "f32", "data": "synthetic (seeded generator, zillumgl_amd/scene.py)",
         "config": {"workload": text, "width": W, "height": H, "spp": spp, "depth": depth,
                    "triangles": int(ps.indices.shape[0]),
-                   "equal_rmse": "GPU and CPU run the same estimator on the same Sobol indices; frames agree "
-                                 "per pixel within 1e-4 (tests/test_gpu_parity.py)"},
+                   "equal_rmse": "GPU and CPU run the same estimator on the same Sobol indices; frames are "
+                                 "equal bit for bit (tests/test_gpu_parity.py)"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "adjoint": adjoint,
