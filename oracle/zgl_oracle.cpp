@@ -770,7 +770,7 @@ template <class R> struct Ctx {
     R sample1D()
     {
         if (sampler == 0) return rand();
-        int idx = sampleOffset + dim;
+        int idx = (int)((uint32_t)sampleOffset + (uint32_t)dim);  // GLSL int addition wraps
         uint32_t r = 0;
         if (idx >= 0 && idx < 131072 * 256) r = sobol().sample((uint32_t)(idx >> 8), idx & 255);
         cn.bytes += 4;
@@ -2157,7 +2157,7 @@ template <class R> struct Ctx {
         uint32_t ux = (uint32_t)tx, uy = (uint32_t)ty;
         randSeed = ux * fc + uy;
         sampleSeed = ux * uy;
-        int off = spp * 256;
+        int off = (int)((uint32_t)spp * 256u);  // uSpp * uSampleDim, wrapping as GLSL int arithmetic
         if (off > 131072 * 256) off -= 131072 * 256;
         sampleOffset = off;
         dim = 0;

@@ -119,3 +119,22 @@ def test_log_exp_pow():
     b = rng.uniform(0.0, 1.0, 10_000).astype(np.float32)
     rel = np.abs(orc.fmath("pow", a, b) / np.power(a.astype(np.float64), b.astype(np.float64)) - 1)
     assert rel.max() < 1e-5
+
+
+def test_sample_offset_wraps_as_glsl_int_arithmetic():
+    """sampleOffset = uSpp * uSampleDim, less uSampleNum * uSampleDim past it
+    (path_integ_naive.glsl:163-164), in GLSL int arithmetic, which wraps modulo 2^32: pass
+    2^24 has offset 2^32 mod 2^32 = 0, the offset of pass 0, and pass 131073 has
+    131073 * 256 - 131072 * 256 = 256, the offset of pass 1.  With the same free counter
+    the seeds are the same too, so those passes render the same frames bit for bit; an
+    int product that overflowed, or no subtraction, would not."""
+    from zillumgl_amd import scene
+    ps = scene.pack(scene.config1(16, 12))
+    base, _ = orc.render_pixels(ps, orc.params(0, 0, 2, 5, 4))
+    top, _ = orc.render_pixels(ps, orc.params(0, 1 << 24, 2, 5, 4))
+    past, _ = orc.render_pixels(ps, orc.params(0, 1, 1, 5, 4))
+    wrapped, _ = orc.render_pixels(ps, orc.params(0, 131073, 1, 5, 4))
+    assert np.array_equal(base.view(np.uint32), top.view(np.uint32))
+    assert np.array_equal(past.view(np.uint32), wrapped.view(np.uint32))
+    other, _ = orc.render_pixels(ps, orc.params(0, 131072, 1, 5, 4))  # offset 2^25: past the table
+    assert not np.array_equal(other.view(np.uint32), past.view(np.uint32))

@@ -153,7 +153,8 @@ ZGL_D uint32_t sobolValue(const DevScene& s, int idx)
 template <bool C> ZGL_D float sample1D(const DevScene& s, Rng& r, Counters& cn)
 {
     if (s.sampler == 0) return rand01(r);
-    uint32_t v = sobolValue(s, r.sampleOffset + r.dim);
+    // sampleOffset + dim, wrapping as GLSL int addition does
+    uint32_t v = sobolValue(s, (int)((uint32_t)r.sampleOffset + (uint32_t)r.dim));
     cnt<C>(cn, 4);
     v ^= r.sampleSeed;
     r.sampleSeed = hashU(r.sampleSeed);
