@@ -105,8 +105,12 @@ def adjoint_bench(hip, steps):
     512 x 512, 32 spp, depth 6, gradients of an L2 image loss against a target rendered
     at parameters perturbed by N(0, 0.1) (clamped to [0, 1]) w.r.t. 160 material
     parameters and 98,304 environment texel channels.  t_forward is zgl_render of the
-    same passes; t_adjoint is zgl_adjoint (its own forward replay, the adjoint replay,
-    the upload of dLoss/dframe and the download of the gradients)."""
+    same passes; t_adjoint is zgl_adjoint of passes not rendered before (its own logged
+    forward pass, the adjoint replay, the upload of dLoss/dframe and the download of the
+    gradients); t_image is zgl_render_replay at the parameters (logged forward pass and
+    value replay); t_step is one optimisation step, the image and then the gradients of
+    the same passes (the adjoint reuses the image's traversal log).  Each timed call takes
+    passes of its own, as an optimisation with fresh samples per step would."""
     import numpy as np
     from zillumgl_amd import scene, render, diff, _lib
     ps = scene.pack(scene.config5())
@@ -117,10 +121,6 @@ def adjoint_bench(hip, steps):
     rng = np.random.default_rng(5)
     p0, e0 = diff.scene_parameters(ps)
     pt = np.clip(p0 + rng.normal(0, 0.1, p0.shape), 0, 1).astype(np.float32)
-    rd = diff.Renderer(ctx, ps, spp, depth)
-    target = rd.image(pt, e0)
-    img = rd.image(p0, e0)
-    d_image = 2.0 * (img - target)
 
     def timed(fn):
         ms = []
@@ -133,17 +133,40 @@ def adjoint_bench(hip, steps):
         return sorted(ms[1:])[len(ms[1:]) // 2]
 
     t_fwd = timed(lambda: ctx.render(_lib.INTEGRATOR_NAIVE, 0, spp, 1, depth))
+    rd = diff.Renderer(ctx, ps, spp, depth)
+    target = rd.image(pt, e0)
+    img = rd.image(p0, e0)
+    d_image = 2.0 * (img - target)
     gm = ge = None
+    fresh = iter(range(1, 10000))
+    n_mat, env_shape = p0.shape[0], e0.shape[:2]
+    d_frame = (np.asarray(d_image, np.float32) / np.float32(spp))
 
     def adj():
         nonlocal gm, ge
-        gm, ge = rd.gradients(p0, e0, d_image)
+        k = next(fresh) * spp
+        gm, ge = ctx.adjoint(d_frame, n_mat, env_shape, k, spp, 1 + k, depth)
+    ctx.set_eval_params(p0, e0)
     t_adj = timed(adj)
+
+    def image():
+        k = next(fresh) * spp
+        ctx.clear()
+        ctx.render_replay(k, spp, 1 + k, depth)
+    t_img = timed(image)
+
+    def step():
+        k = next(fresh) * spp
+        ctx.clear()
+        ctx.render_replay(k, spp, 1 + k, depth)
+        ctx.adjoint(d_frame, n_mat, env_shape, k, spp, 1 + k, depth)
+    t_step = timed(step)
     ctx.close()
     samples = W * H * spp
     return {"metric": "adjoint-gradient Msamples/s", "value": round(samples / (t_adj / 1e3) / 1e6, 3),
             "unit": "Msamples/s", "t_adjoint_ms": round(t_adj, 3), "t_forward_ms": round(t_fwd, 3),
-            "adjoint_over_forward": round(t_adj / t_fwd, 3),
+            "adjoint_over_forward": round(t_adj / t_fwd, 3), "t_image_ms": round(t_img, 3),
+            "t_step_ms": round(t_step, 3), "step_over_forward": round(t_step / t_fwd, 3),
             "gradient_norms": {"materials": float(np.linalg.norm(gm)), "env": float(np.linalg.norm(ge))},
             "workload": "tessellated Cornell box, 201,588 tris, 32 Disney materials, 256x128 env, 512x512, "
                         "32 spp, depth 6, L2 loss vs a target at parameters + N(0, 0.1); 160 material + "
@@ -152,15 +175,13 @@ def adjoint_bench(hip, steps):
 
 def batch_bench(reps):
     """64 scenes (Cornell boxes tessellated to 1k-2M triangles, 128^2-1024^2, 4-64 spp,
-    depth 3-10) rendered four ways: (a) per scene, one launch of the plain kernel (one
-    thread per pixel running its passes, the shape of path_integ_naive.glsl; it is
-    zgl_render_replay at the uploaded parameters); (b) per scene and per pass, one
-    zgl_render launch each -- the reference's dispatch pattern
-    (NaivePathIntegrator::renderOnePass, src/integrator/NaivePath.cpp:77-104, one dispatch
-    per sample pass); (c) per scene, one zgl_render launch holding all its passes; (d) one
-    batched persistent launch replayed from a HIP graph, after the autotuner picked its
-    scene order, regeneration threshold and occupancy.  Frames are bit-identical between
-    the four forms (tests/test_gpu_batch.py), so RMSE is equal."""
+    depth 3-10) rendered three ways: (a) per scene and per pass, one zgl_render launch
+    each -- the reference's dispatch pattern (NaivePathIntegrator::renderOnePass,
+    src/integrator/NaivePath.cpp:77-104, one dispatch per sample pass); (b) per scene, one
+    zgl_render launch holding all its passes; (c) one batched persistent launch replayed
+    from a HIP graph, after the autotuner picked its scene order, regeneration threshold
+    and occupancy.  Frames are bit-identical between the three forms
+    (tests/test_gpu_batch.py), so RMSE is equal."""
     from concurrent.futures import ThreadPoolExecutor
     from zillumgl_amd import scene, render, _lib
     descs = scene.config4()
@@ -188,11 +209,6 @@ def batch_bench(reps):
                 c.render(_lib.INTEGRATOR_NAIVE, first + k, 1, fc + k, depth)
             c.synchronize()
 
-    def plain():
-        for c, (first, n, fc, depth) in zip(ctxs, passes):
-            c.render_replay(first, n, fc, depth)
-            c.synchronize()
-
     def timed(fn, reps=reps):
         fn()
         ts = []
@@ -201,7 +217,6 @@ def batch_bench(reps):
             fn()
             ts.append(time.perf_counter() - t)
         return sorted(ts)[len(ts) // 2]
-    t_plain = timed(plain, 1)
     t_pass = timed(per_pass, 1)
     t_seq = timed(sequential)
     b = render.Batch(0)
@@ -225,8 +240,6 @@ def batch_bench(reps):
     n_launches = sum(n for (_, n, _, _) in passes)
     return {"metric": "batched Msamples/s", "value": round(samples / t_bat / 1e6, 3), "unit": "Msamples/s",
             "t_batch_ms": round(t_bat * 1e3, 3),
-            "t_plain_ms": round(t_plain * 1e3, 3), "plain_value": round(samples / t_plain / 1e6, 3),
-            "speedup_vs_plain": round(t_plain / t_bat, 3),
             "t_per_pass_ms": round(t_pass * 1e3, 3), "per_pass_launches": n_launches,
             "per_pass_value": round(samples / t_pass / 1e6, 3),
             "speedup_vs_per_pass": round(t_pass / t_bat, 3),
@@ -237,8 +250,8 @@ def batch_bench(reps):
                       "tuning_s": round(t_tune, 1)},
             "workload": f"64 Cornell-box scenes, {tris:,} triangles in total (1k-2M each, log-uniform), "
                         f"128^2-1024^2, spp 4-64, depth 3-10, {samples / 1e6:.1f} M samples per launch; "
-                        "frames bit-identical between the four forms; plain = one launch per scene of the "
-                        "one-thread-per-pixel kernel, per_pass = one persistent launch per scene and pass "
+                        "frames bit-identical between the three forms; per_pass = one persistent launch per "
+                        "scene and pass "
                         "(the reference's dispatch pattern), sequential = one persistent launch per scene"}
 
 
@@ -486,13 +499,14 @@ def main():
     adjoint = None
     if not args.no_adjoint and args.workload == "c2":
         adjoint = adjoint_bench(hip, max(args.steps, 3))
-        print(f"[bench] adjoint {adjoint['value']} Msamples/s, {adjoint['adjoint_over_forward']}x forward",
+        print(f"[bench] adjoint {adjoint['value']} Msamples/s, {adjoint['adjoint_over_forward']}x forward; "
+              f"image {adjoint['t_image_ms']} ms, step {adjoint['t_step_ms']} ms ({adjoint['step_over_forward']}x forward)",
               file=sys.stderr, flush=True)
 
     batch = None
     if not args.no_batch and args.workload == "c2":
         batch = batch_bench(max(args.steps, 3))
-        print(f"[bench] batch {batch['value']} Msamples/s, {batch['speedup_vs_plain']}x plain per-scene launches, "
+        print(f"[bench] batch {batch['value']} Msamples/s, "
               f"{batch['speedup_vs_per_pass']}x per-pass launches, "
               f"{batch['speedup_vs_sequential']}x per-scene persistent launches",
               file=sys.stderr, flush=True)

@@ -244,7 +244,12 @@ zgl_status zgl_reset_counters(zgl_context* ctx);
 zgl_status zgl_set_eval_params(zgl_context* ctx, const float* material_params /* n_materials x 5 */,
                                const float* env_rgb /* env_height x env_width x 3 */);
 /* The estimator at the evaluation point, added into the frame as zgl_render adds
- * (naive integrator only).  At the uploaded parameters it is zgl_render's, bit for bit. */
+ * (naive integrator only).  At the uploaded parameters it is zgl_render's, bit for bit.
+ * Works in chunks of passes like zgl_adjoint: the forward tracer logs each sample's
+ * traversal results, and each sample is evaluated from the log.  The context keeps a log
+ * that holds all the passes of a call: a zgl_render_replay or zgl_adjoint of the same
+ * passes (first_pass, n_passes, free_counter, max_depth and the sampling options) made
+ * before the scene, camera, noise or textures change skips the forward tracer. */
 zgl_status zgl_render_replay(zgl_context* ctx, const zgl_pass_params* params);
 /* Path-replay adjoint: gradients of sum(d_frame . frame sum of these passes) with
  * respect to the parameters at the evaluation point.  d_frame: W*H*4 floats (RGBA
@@ -253,6 +258,7 @@ zgl_status zgl_render_replay(zgl_context* ctx, const zgl_pass_params* params);
  * Works in chunks of passes: the forward tracer logs each sample's traversal results
  * (up to 16 queries, 128 bytes per sample, with the sample values at most 8 GiB per
  * chunk) and the replay differentiates from the log. */
+/* The traversal log is kept and reused as zgl_render_replay's is. */
 zgl_status zgl_adjoint(zgl_context* ctx, const zgl_pass_params* params, const float* d_frame,
                        float* d_material_params, float* d_env_rgb);
 

@@ -160,6 +160,47 @@ def test_adjoint_deep_paths_match_oracle_fp64(ctx, c2s):
     assert em <= 1e-4 and ee <= 1e-4
 
 
+def test_adjoint_reuses_the_image_log_only_for_its_scene_and_passes(ctx, c2s):
+    """zgl_render_replay logs every sample's traversal results; a zgl_adjoint of the same
+    passes right after differentiates from that log, and after another scene is uploaded
+    (or other passes are asked for) it logs its own.  Each against the oracle's FP64
+    adjoint: a log of other rays would give other gradients."""
+    print("[gpu] adjoint: traversal log of the image reused, and not across scenes", flush=True)
+    rng = np.random.default_rng(8)
+    d_frame = rng.uniform(-1, 1, (c2s.height, c2s.width, 3)).astype(np.float32)
+    _, _, p1, e1 = _perturb(c2s, 8)
+    prm = orc.params(0, 0, PASSES, 1, DEPTH)
+    ctx.upload(c2s)
+    ctx.set_eval_params(p1, e1)
+    ctx.clear()
+    ctx.render_replay(0, PASSES, 1, DEPTH)
+    gm, ge = ctx.adjoint(d_frame, p1.shape[0], e1.shape, 0, PASSES, 1, DEPTH)
+    rm, re = orc.adjoint(c2s, prm, _records(c2s, p1), e1, d_frame, fp32_values=True)
+    em, _ = _close(gm, rm, 1e-3)
+    ee, _ = _close(ge, re, 1e-3)
+    print(f"[gpu] adjoint after the image: material {em:.3g}, env {ee:.3g}", flush=True)
+    assert em <= 1e-4 and ee <= 1e-4
+    # the image's log of this scene, then another scene of the same size
+    other = scene.pack(scene.config2(width=96, height=54, n_spheres=6, subdiv=3, env_size=(256, 128), seed=12))
+    ctx.clear()
+    ctx.render_replay(0, PASSES, 1, DEPTH)
+    ctx.upload(other)
+    p0o, e0o = diff.scene_parameters(other)
+    gm2, ge2 = ctx.adjoint(d_frame, p0o.shape[0], e0o.shape, 0, PASSES, 1, DEPTH)
+    rm2, re2 = orc.adjoint(other, prm, _records(other, p0o), e0o, d_frame, fp32_values=True)
+    em2, _ = _close(gm2, rm2, 1e-3)
+    ee2, _ = _close(ge2, re2, 1e-3)
+    print(f"[gpu] adjoint of another scene: material {em2:.3g}, env {ee2:.3g}", flush=True)
+    assert em2 <= 1e-4 and ee2 <= 1e-4
+    # other passes of the same scene: the log of passes [0, 3) is not theirs
+    gm3, ge3 = ctx.adjoint(d_frame, p0o.shape[0], e0o.shape, 4, PASSES, 5, DEPTH)
+    rm3, re3 = orc.adjoint(other, orc.params(0, 4, PASSES, 5, DEPTH), _records(other, p0o), e0o, d_frame,
+                           fp32_values=True)
+    em3, _ = _close(gm3, rm3, 1e-3)
+    ee3, _ = _close(ge3, re3, 1e-3)
+    assert em3 <= 1e-4 and ee3 <= 1e-4
+
+
 def test_autograd_l2_gradient_matches_central_differences(ctx, c2s):
     print("[gpu] adjoint: torch autograd L2 loss vs central differences", flush=True)
     ctx.upload(c2s)

@@ -1,7 +1,8 @@
 """GPU parity of the batched persistent launch: one graph launch over several contexts
 writes into every frame exactly (bit for bit) what zgl_render of the same passes writes,
 whatever the scene order, regeneration threshold and blocks per CU, and after tuning;
-so do the plain and per-pass forms it is timed against."""
+so do the per-pass form it is timed against and the differentiable mode's replay at the
+uploaded parameters."""
 import numpy as np
 import pytest
 
@@ -73,11 +74,11 @@ def test_batch_equals_per_scene_launches(contexts, batch_scenes):
     b.close()
 
 
-def test_plain_and_per_pass_forms_equal_per_scene_launches(contexts, batch_scenes):
-    """The other forms the batched launch is timed against write the same frames: the
-    plain one-thread-per-pixel kernel (zgl_render_replay at the uploaded parameters) and
-    one zgl_render per pass."""
-    print("[gpu] batch: plain and per-pass forms vs zgl_render", flush=True)
+def test_replay_and_per_pass_forms_equal_per_scene_launches(contexts, batch_scenes):
+    """zgl_render_replay at the uploaded parameters (the logged forward pass and the
+    replay from its log) and one zgl_render per pass write the frames zgl_render of all the
+    passes writes."""
+    print("[gpu] batch: replay and per-pass forms vs zgl_render", flush=True)
     refs = _reference(contexts, batch_scenes)
     for k, (c, (ps, spp, depth, first)) in enumerate(zip(contexts, batch_scenes)):
         c.clear()

@@ -34,7 +34,7 @@ static int defaultThreshold(WalkKind kind, int bvhSize)
     default: return std::min(28, std::max(4, (int)std::lround(1.4 * l - 2.0)));
     }
 }
-// Largest sample-value plus traversal-log buffer of one adjoint chunk of passes.
+// Largest sample-value plus traversal-log buffer of one chunk of the differentiable mode's passes.
 constexpr size_t kAdjointLogBytes = (size_t)8 << 30;
 // Largest total of a batch's per-sample value buffers.
 constexpr size_t kBatchSampleBytes = (size_t)64 << 30;
@@ -155,7 +155,11 @@ struct zgl_context {
     DevBuf tex, texScale, texRaw;  // texture array (linear RGB float4), per-layer uv scale, upload staging
     int maxTexId = -1;             // largest texture id the uploaded scene's triangles use
     DevBuf naiveRes;  // per-sample values of the persistent naive tracer
-    DevBuf hitLog;    // per-sample traversal results for the adjoint
+    DevBuf hitLog;    // per-sample traversal results of the differentiable mode's passes
+    // The passes whose traversal results hitLog holds in full and the generation they were
+    // logged at (0: none): the results depend only on the uploaded scene and the passes.
+    uint64_t logGeneration = 0;
+    PassParams logPasses = {};
     DevBuf refSum;    // FP64 reference: per-pixel RGB sums of the passes rendered so far
     int wfTraceBlocksPerCU = 0, wfShadeBlocksPerCU = 0;
     DeviceBvh bvh;     // work memory of the device MTBVH build
@@ -947,6 +951,56 @@ zgl_status zgl_set_eval_params(zgl_context* ctx, const float* material_params, c
     });
 }
 
+}  // extern "C"
+
+// Whether hitLog holds the traversal results of exactly these passes of the current scene.
+static bool logHolds(const zgl_context* ctx, const PassParams& pp)
+{
+    const PassParams& l = ctx->logPasses;
+    return ctx->logGeneration != 0 && ctx->logGeneration == ctx->generation && l.firstPass == pp.firstPass &&
+           l.nPasses == pp.nPasses && l.freeCounter == pp.freeCounter && l.maxDepth == pp.maxDepth &&
+           l.russianRoulette == pp.russianRoulette && l.sampleLight == pp.sampleLight &&
+           l.lightEnvUniform == pp.lightEnvUniform && l.lightPortion == pp.lightPortion;
+}
+
+// The differentiable mode's passes in chunks whose traversal log fits kAdjointLogBytes:
+// for each chunk the persistent forward kernel logs every sample's traversal results (the
+// rays depend only on the uploaded scene, which all sampling uses), then `replay` runs the
+// chunk's samples from the log.  A request that fits one chunk and whose passes the log
+// already holds (a zgl_adjoint after zgl_render_replay of the same passes) skips the
+// forward kernel; such a log is kept for the next call.
+template <class Replay>
+static void loggedPasses(zgl_context* ctx, const DevScene& s, const PassParams& pp, int regenThreshold,
+                         Replay replay)
+{
+    if (!ctx->naiveBlocksPerCU) ctx->naiveBlocksPerCU = naivePersistentBlocksPerCU();
+    const size_t px = (size_t)ctx->filmW * ctx->filmH;
+    const size_t perPass = px * (16 + (size_t)kHitLogQueries * 8);
+    const int chunk = (int)std::max<size_t>(1, std::min<size_t>(pp.nPasses, kAdjointLogBytes / perPass));
+    const bool reuse = chunk == pp.nPasses && logHolds(ctx, pp);
+    ctx->logGeneration = 0;
+    ctx->ensure(ctx->naiveRes, px * 16 * chunk, "sample values");
+    ctx->ensure(ctx->hitLog, px * kHitLogQueries * 8 * chunk, "traversal log");
+    const int threshold = regenThreshold > 0 ? regenThreshold : defaultThreshold(WalkKind::Naive, s.bvhSize);
+    for (int k0 = 0; k0 < pp.nPasses; k0 += chunk) {
+        PassParams q = pp;
+        q.firstPass = pp.firstPass + k0;
+        q.freeCounter = pp.freeCounter + k0;
+        q.nPasses = std::min(chunk, pp.nPasses - k0);
+        if (!reuse)
+            hipCheck(launchNaiveLogged(s, q, (float4*)ctx->naiveRes.p, (int2*)ctx->hitLog.p, (int*)ctx->workCounter.p,
+                                       threshold, ctx->numCUs * ctx->naiveBlocksPerCU, ctx->stream),
+                     "logged forward pass");
+        replay(q);
+    }
+    if (chunk == pp.nPasses) {
+        ctx->logPasses = pp;
+        ctx->logGeneration = ctx->generation;
+    }
+}
+
+extern "C" {
+
 zgl_status zgl_render_replay(zgl_context* ctx, const zgl_pass_params* p)
 {
     return guarded(ctx, [&] {
@@ -955,9 +1009,13 @@ zgl_status zgl_render_replay(zgl_context* ctx, const zgl_pass_params* p)
         preparePasses(ctx, p, s, pp);
         if (p->integrator != ZGL_INTEGRATOR_NAIVE) invalid("the replay estimator is the naive integrator's");
         if (ctx->maxTexId >= 0) invalid("the differentiable mode does not take textured materials");
-        hipCheck(launchNaiveReplay(s, pp, (const float4*)ctx->evalMat.p, (const float4*)ctx->evalEnv.p,
-                                   (float4*)ctx->frame.p, nullptr, nullptr, nullptr, 1, nullptr, nullptr, ctx->stream),
-                 "replay pass");
+        // each chunk's samples are added to the frame in pass order, after the previous chunk's
+        loggedPasses(ctx, s, pp, p->regen_threshold, [&](const PassParams& q) {
+            hipCheck(launchNaiveReplay(s, q, (const float4*)ctx->evalMat.p, (const float4*)ctx->evalEnv.p,
+                                       (float4*)ctx->frame.p, nullptr, nullptr, nullptr, 1, nullptr,
+                                       (const int2*)ctx->hitLog.p, ctx->stream),
+                     "replay pass");
+        });
     });
 }
 
@@ -984,29 +1042,13 @@ zgl_status zgl_adjoint(zgl_context* ctx, const zgl_pass_params* p, const float* 
         double* envSum = envCopies + (size_t)copies * ne;
         hipCheck(launchFill((float*)ctx->dMat.p, (int64_t)nm * 2, 0.0f, ctx->stream), "clear");
         hipCheck(launchFill((float*)envCopies, (int64_t)ne * 2 * copies, 0.0f, ctx->stream), "clear");
-        // In chunks of passes: the persistent forward kernel logs every sample's traversal
-        // results (the rays depend only on the uploaded scene, which all sampling uses),
-        // then the replay kernel differentiates the chunk's samples from the log.
-        if (!ctx->naiveBlocksPerCU) ctx->naiveBlocksPerCU = naivePersistentBlocksPerCU();
-        const size_t perPass = px * (16 + (size_t)kHitLogQueries * 8);
-        const int chunk = (int)std::max<size_t>(1, std::min<size_t>(pp.nPasses, kAdjointLogBytes / perPass));
-        ctx->ensure(ctx->naiveRes, px * 16 * chunk, "sample values");
-        ctx->ensure(ctx->hitLog, px * kHitLogQueries * 8 * chunk, "traversal log");
-        for (int k0 = 0; k0 < pp.nPasses; k0 += chunk) {
-            PassParams q = pp;
-            q.firstPass = pp.firstPass + k0;
-            q.freeCounter = pp.freeCounter + k0;
-            q.nPasses = std::min(chunk, pp.nPasses - k0);
-            hipCheck(launchNaiveLogged(s, q, (float4*)ctx->naiveRes.p, (int2*)ctx->hitLog.p, (int*)ctx->workCounter.p,
-                                       p->regen_threshold > 0 ? p->regen_threshold
-                                                              : defaultThreshold(WalkKind::Naive, s.bvhSize),
-                                       ctx->numCUs * ctx->naiveBlocksPerCU, ctx->stream),
-                     "logged forward pass");
+        // the chunks' gradients, differentiated from the traversal log
+        loggedPasses(ctx, s, pp, p->regen_threshold, [&](const PassParams& q) {
             hipCheck(launchNaiveReplay(s, q, (const float4*)ctx->evalMat.p, (const float4*)ctx->evalEnv.p,
                                        (float4*)ctx->frame.p, (const float4*)ctx->dFrame.p, (double*)ctx->dMat.p,
                                        envCopies, copies, envSum, (const int2*)ctx->hitLog.p, ctx->stream),
                      "adjoint pass");
-        }
+        });
         std::vector<double> gm(nm), ge(ne);
         hipCheck(hipMemcpyAsync(gm.data(), ctx->dMat.p, nm * 8, hipMemcpyDeviceToHost, ctx->stream), "copy");
         hipCheck(hipMemcpyAsync(ge.data(), envSum, ne * 8, hipMemcpyDeviceToHost, ctx->stream), "copy");

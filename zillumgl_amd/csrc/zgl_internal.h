@@ -57,6 +57,15 @@ hipError_t launchTraceAny(const DevScene& s, const float* rays, const float* tma
 hipError_t launchNaivePersistent(const DevScene& s, const PassParams& p, float4* frame, float4* res, int resPasses,
                                  int* pixelCounter, int threshold, int blocks, unsigned long long* counters,
                                  hipStream_t st);
+// Workgroups of `threads` threads of `kernel` that one compute unit holds at once: the
+// larger of `queried` (the runtime's occupancy query) and the residency computed from the
+// kernel's VGPR count and static LDS for a CDNA3/4 compute unit (4 SIMDs, 512 VGPRs per
+// lane allocated in granules of 8, at most 8 waves per SIMD, 160 KiB of LDS).  ROCm
+// 7.2's runtime answered 2 for a 124-VGPR kernel of 256 threads that holds 4 per CU (so
+// the ROCm 7.0 runtime, the measured launch times and the wave counters), and a process
+// may bind the library to either runtime.  Launching more workgroups than fit is safe for
+// these kernels (no workgroup waits on another); fewer leaves SIMDs idle.
+int residentBlocksPerCU(const void* kernel, int threads, int queried);
 int naivePersistentBlocksPerCU();
 // The triple tracer's camera pass on the persistent schedule: p.nPasses passes in chunks
 // of resPasses (res holds resPasses x W x H values), added to the planar film in order.

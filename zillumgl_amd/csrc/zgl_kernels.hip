@@ -515,6 +515,20 @@ hipError_t launchSrgbExpand(const uint8_t* rgb, const float* lut, float4* out, i
     hipLaunchKernelGGL(k_srgb_expand, dim3(gridFor(n, 256)), dim3(256), 0, st, rgb, lut, out, n);
     return hipGetLastError();
 }
+int residentBlocksPerCU(const void* kernel, int threads, int queried)
+{
+    int computed = 0;
+    hipFuncAttributes a{};
+    if (hipFuncGetAttributes(&a, kernel) == hipSuccess && a.numRegs > 0) {
+        const int alloc = (a.numRegs + 7) / 8 * 8;
+        const int wavesPerSimd = std::min(8, 512 / alloc);
+        const int wavesPerBlock = (threads + 63) / 64;
+        computed = std::max(1, wavesPerSimd * 4 / wavesPerBlock);
+        if (a.sharedSizeBytes > 0) computed = std::min<int>(computed, (int)(((size_t)160 << 10) / a.sharedSizeBytes));
+    }
+    return std::max({1, queried, computed});
+}
+
 hipError_t launchFill(float* p, int64_t n, float v, hipStream_t st)
 {
     if (n <= 0) return hipSuccess;

@@ -294,8 +294,9 @@ int lightPersistentBlocksPerCU(bool triple)
     const hipError_t e =
         triple ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_lightpath_persistent<true>, 256, 0)
                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_lightpath_persistent<false>, 256, 0);
-    if (e != hipSuccess) return 1;
-    return n > 0 ? n : 1;
+    if (e != hipSuccess) n = 0;
+    return triple ? residentBlocksPerCU(reinterpret_cast<const void*>(k_lightpath_persistent<true>), 256, n)
+                  : residentBlocksPerCU(reinterpret_cast<const void*>(k_lightpath_persistent<false>), 256, n);
 }
 
 hipError_t launchLightPersistent(const DevScene& s, const PassParams& p, bool triple, float* film, int* counter,

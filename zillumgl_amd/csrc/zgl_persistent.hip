@@ -854,22 +854,22 @@ hipError_t launchTriplePersistent(const DevScene& s, const PassParams& p, float*
 int triplePersistentBlocksPerCU()
 {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_triple_persistent, 256, 0) != hipSuccess) return 1;
-    return n > 0 ? n : 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_triple_persistent, 256, 0) != hipSuccess) n = 0;
+    return residentBlocksPerCU(reinterpret_cast<const void*>(k_triple_persistent), 256, n);
 }
 
 int naiveBatchBlocksPerCU()
 {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_naive_batch, 256, 0) != hipSuccess) return 1;
-    return n > 0 ? n : 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_naive_batch, 256, 0) != hipSuccess) n = 0;
+    return residentBlocksPerCU(reinterpret_cast<const void*>(k_naive_batch), 256, n);
 }
 
 int naivePersistentBlocksPerCU()
 {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_naive_persistent<false, false>, 256, 0) != hipSuccess) return 1;
-    return n > 0 ? n : 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_naive_persistent<false, false>, 256, 0) != hipSuccess) n = 0;
+    return residentBlocksPerCU(reinterpret_cast<const void*>(k_naive_persistent<false, false>), 256, n);
 }
 
 }  // namespace zgl

@@ -231,9 +231,8 @@ size_t globalQueueBytes(int w, int h) { return (size_t)2 * w * h * sizeof(QueueI
 int globalQueueBlocks()
 {
     int perCU = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_gq_extend, kQBlock, 0) != hipSuccess || perCU < 1)
-        perCU = 1;
-    return perCU;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_gq_extend, kQBlock, 0) != hipSuccess) perCU = 0;
+    return residentBlocksPerCU(reinterpret_cast<const void*>(k_gq_extend), kQBlock, perCU);
 }
 
 hipError_t launchGlobalQueue(const DevScene& s, const PassParams& p, float4* frame, void* queues, int* counts,

@@ -500,10 +500,10 @@ int wavefrontBatch(int w, int h, int nPasses, size_t budget)
 void wavefrontBlocks(int& traceBlocksPerCU, int& shadeBlocksPerCU)
 {
     int a = 0, b = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_wf_trace, kWfBlock, 0) != hipSuccess || a < 1) a = 1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_wf_shade, kWfBlock, 0) != hipSuccess || b < 1) b = 1;
-    traceBlocksPerCU = a;
-    shadeBlocksPerCU = b;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_wf_trace, kWfBlock, 0) != hipSuccess) a = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_wf_shade, kWfBlock, 0) != hipSuccess) b = 0;
+    traceBlocksPerCU = residentBlocksPerCU(reinterpret_cast<const void*>(k_wf_trace), kWfBlock, a);
+    shadeBlocksPerCU = residentBlocksPerCU(reinterpret_cast<const void*>(k_wf_shade), kWfBlock, b);
 }
 
 hipError_t launchNaiveWavefront(const DevScene& s, const PassParams& p, float4* frame, void* work, int batch,
